@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Benchmark: HBM bandwidth of the streamed complex128 gate sweep on one MI355X.
+
+Workload (one step): an n = 30 qubit complex128 state (16 GiB in HBM); a Haar-random 2x2
+gate on every qubit 0..29, then a CNOT ladder (q, q+1), q = 0..28 -- 59 gates, each one
+streamed pass over the state, launched one call at a time through the quantum_sim_v3
+StateVector surface (apply_gate_cache_optimized / cnot), exactly as a reference user
+drives it.  Inputs stay resident in HBM for the whole run.
+
+value     = whole-job algorithmic HBM bytes / wall time (GB/s): 32 * 2^n per one-qubit
+            gate, 16 * 2^n per CNOT (only the control = 1 half moves).
+roofline  = the dominant kernel (the one-qubit gate): 32 * 2^n bytes / its average launch
+            duration, timed with HIP events recorded on the state's own stream around every
+            one-qubit launch inside the timed region; peak 8000 GB/s.
+cpu_baseline = oracle/cpu_baseline.py in a subprocess (reference v3 loop on host cores).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--qubits 30] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "lightning-lite-quantum-simulator_amd")
+sys.path.insert(0, PKG)
+
+METRIC = "HBM GB/s (% of peak) gate sweeps @30q c128; adjoint grads/s vs param-shift"
+PEAK_HBM_GBPS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n):
+    cmd = [sys.executable, os.path.join(ROOT, "oracle", "cpu_baseline.py"), "--qubits", str(n)]
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+        sys.stderr.write(r.stderr)
+        if r.returncode != 0:
+            return {"error": f"cpu_baseline exited {r.returncode}"}
+        return json.loads(r.stdout.strip().splitlines()[-1])
+    except Exception as e:  # reported, never fatal for the GPU measurement
+        return {"error": str(e)}
+
+
+def load_traffic(n):
+    path = os.path.join(ROOT, "profiles", f"traffic_apply_1q_{n}q.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {a.gpus} is not supported (the state-vector "
+              "path is single-device here; sharding is a design note, not a run)", file=sys.stderr)
+        sys.exit(2)
+
+    import quantum_sim_v3 as V3
+    from llsv import circuits as C
+
+    n = a.qubits
+    size = 1 << n
+    ops = C.haar_cnot_sweep(0, n, 1)
+    one_q = [(op[1], list(op[2])) for op in ops if op[0] == "u"]
+    cnots = [(op[1], op[2]) for op in ops if op[0] == "cnot"]
+    bytes_1q = 32.0 * size
+    bytes_cx = 16.0 * size
+    bytes_step = bytes_1q * len(one_q) + bytes_cx * len(cnots)
+
+    log(f"allocating {n}-qubit state ({16 * size / 2**30:.1f} GiB)")
+    sv = V3.StateVector(n)
+
+    def step(marks=None):
+        slot = 0
+        for q, g in one_q:
+            if marks is not None:
+                sv.mark(slot)
+            sv.apply_gate_cache_optimized(q, g)
+            if marks is not None:
+                sv.mark(slot + 1)
+                slot += 2
+        cx0 = slot
+        for c, t in cnots:
+            if marks is not None:
+                sv.mark(slot)
+            sv.cnot(c, t)
+            if marks is not None:
+                sv.mark(slot + 1)
+                slot += 2
+        if marks is not None:
+            marks["1q"] += [sv.mark_elapsed(2 * i, 2 * i + 1) for i in range(len(one_q))]
+            marks["cx"] += [sv.mark_elapsed(cx0 + 2 * i, cx0 + 2 * i + 1) for i in range(len(cnots))]
+
+    for w in range(a.warmup):
+        step()
+        sv.sync()
+        log(f"warmup {w + 1}/{a.warmup}")
+
+    marks = {"1q": [], "cx": []}
+    sv.sync()
+    t0 = time.perf_counter()
+    sv.timer_start()
+    for k in range(a.steps):
+        step(marks)
+        if (k + 1) % max(1, a.steps // 5) == 0:
+            log(f"step {k + 1}/{a.steps}")
+    dev_ms = sv.timer_stop()
+    sv.sync()
+    wall = time.perf_counter() - t0
+    norm = sv.norm()
+    log(f"timed {a.steps} steps: wall {wall:.3f} s, device {dev_ms / 1e3:.3f} s, norm-1 {norm - 1:.2e}")
+
+    avg_1q = sum(marks["1q"]) / len(marks["1q"]) / 1e3
+    avg_cx = sum(marks["cx"]) / len(marks["cx"]) / 1e3
+    achieved = bytes_1q / avg_1q / 1e9
+    value = bytes_step * a.steps / wall / 1e9
+    gates = (len(one_q) + len(cnots)) * a.steps
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{n}-qubit complex128 sweep: Haar 2x2 on every qubit then CNOT ladder (q, q+1); "
+                         f"{len(one_q) + len(cnots)} streamed gates per step, one pass each"),
+            "n_qubits": n,
+            "state_GiB": 16 * size / 2**30,
+            "gates_per_step": len(one_q) + len(cnots),
+            "bytes_per_step": bytes_step,
+            "parallelism": "single GPU",
+        },
+        "pct_of_peak": round(100 * value / PEAK_HBM_GBPS, 1),
+        "gates_per_s": round(gates / wall, 1),
+        "device_GBps": round(bytes_step * a.steps / (dev_ms / 1e3) / 1e9, 1),
+        "kernels": {
+            "apply_1q": {"avg_ms": round(avg_1q * 1e3, 4), "GBps": round(achieved, 1),
+                         "launches": len(marks["1q"])},
+            "cnot": {"avg_ms": round(avg_cx * 1e3, 4), "GBps": round(bytes_cx / avg_cx / 1e9, 1),
+                     "launches": len(marks["cx"])},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": PEAK_HBM_GBPS,
+            "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBPS, 4),
+            "traffic": load_traffic(n),
+            "kernel": "k_apply_1q (32 * 2^n bytes per launch)",
+        },
+        "norm_minus_1": norm - 1,
+    }
+    del sv
+    if not a.no_cpu_baseline:
+        log("cpu baseline (subprocess)")
+        line["cpu_baseline"] = cpu_baseline(n)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

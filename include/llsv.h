@@ -1,0 +1,167 @@
+/*
+ * llsv.h -- C ABI of the MI355X (gfx950) complex128 state-vector engine.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, no torch or HIP types in
+ * the signatures (streams are passed as void*).  Every function returns
+ * LLSV_OK (0) on success or a negative LLSV_ERR_* code; the message of the last
+ * failure on the calling thread is llsv_last_error().  No C++ exception crosses
+ * this boundary.
+ *
+ * Conventions (the reference's, src/Backend.h:3-8):
+ *   - a state of n qubits is 2^n complex128 amplitudes, interleaved (re, im);
+ *   - qubit q is bit q of the basis index (little endian);
+ *   - a one-qubit gate is the row-major 2x2 g = {g00, g01, g10, g11}, passed as
+ *     8 doubles {g00re, g00im, g01re, g01im, g10re, g10im, g11re, g11im}:
+ *     a0' = g00 a0 + g01 a1, a1' = g10 a0 + g11 a1 (src/StateVectorOptimized.cpp:71-75);
+ *   - a dense k-qubit block is 2^k x 2^k complex row-major; operator bit j <->
+ *     targets[j] (src/Noise.h:58-65).
+ *
+ * Work is asynchronous on the state's own HIP stream; calls that return host
+ * values (norm, expectations, copies) synchronise that stream.  A state must not
+ * be used from two threads at once (as in the reference, which holds the GIL).
+ */
+#ifndef LLSV_H
+#define LLSV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LLSV_OK 0
+#define LLSV_ERR_ARG (-1)    /* bad argument: the reference's std::invalid_argument / runtime_error */
+#define LLSV_ERR_NOMEM (-2)  /* device or pinned-host allocation failed */
+#define LLSV_ERR_DEVICE (-3) /* HIP runtime error */
+#define LLSV_ERR_HANDLE (-4) /* null handle */
+
+#define LLSV_MAX_QUBITS 34   /* 2^34 x 16 B = 256 GiB, the most one 288 GB HBM3E device holds */
+#define LLSV_MAX_DENSE_K 5   /* largest fused block: 32 x 32 */
+
+typedef struct llsv_state llsv_state;
+
+/* ---- lifetime ------------------------------------------------------------ */
+
+/* New state |0...0> on the current HIP device.
+ * Replaces StateVectorOptimized(int n), src/StateVectorOptimized.cpp:17-24. */
+int llsv_create(int n_qubits, llsv_state** out);
+
+/* New state holding a copy of n_amps host amplitudes; n_amps must be a power of two.
+ * Replaces StateVectorOptimized(py::array_t<complex<double>>), src/StateVectorOptimized.cpp:26-47
+ * (the reference borrows the buffer; here the device copy is the state and the caller
+ * keeps host and device coherent with llsv_copy_from_host / llsv_copy_to_host). */
+int llsv_create_from_host(const double* amps, uint64_t n_amps, llsv_state** out);
+
+/* Replaces ~StateVectorOptimized, src/StateVectorOptimized.cpp:49-53.  Null is a no-op. */
+int llsv_destroy(llsv_state* s);
+
+/* Back to |0...0>.  Replaces ll::Backend::reset (src/Backend.h:30) for the same n. */
+int llsv_reset(llsv_state* s);
+
+/* get_num_qubits / get_size, src/StateVectorOptimized.cpp:108-109.  -1 / 0 on a null handle. */
+int llsv_num_qubits(const llsv_state* s);
+uint64_t llsv_size(const llsv_state* s);
+
+/* ---- gates ---------------------------------------------------------------- */
+
+/* Dense one-qubit gate on `target` in [0, n).
+ * Replaces StateVectorOptimized::apply_gate_cache_optimized (src/StateVectorOptimized.cpp:56-78),
+ * apply_gate_standard (:81-97) and the v2 apply_gate_optimized (src/bindings_v2.cpp:56-72). */
+int llsv_apply_1q(llsv_state* s, int target, const double g[8]);
+
+/* CNOT: swap amplitudes i <-> i|t for control bit 1, target bit 0.
+ * Replaces ll::Backend::cnot (src/Backend.h:90-97, src/StateVectorBackend.h:109-135);
+ * documented for v3 at README:158 but never bound there. */
+int llsv_apply_cnot(llsv_state* s, int control, int target);
+
+/* CZ = mcz({a, b}) (src/Backend.h:7). */
+int llsv_apply_cz(llsv_state* s, int a, int b);
+
+/* Negate every amplitude whose bits on `qubits` are all 1.
+ * Replaces ll::Backend::mcz (src/Backend.h:99-110, src/StateVectorBackend.h:137-151). */
+int llsv_apply_mcz(llsv_state* s, const int* qubits, int n_qubits);
+
+/* Dense k-qubit block, 1 <= k <= LLSV_MAX_DENSE_K, m = 2^k x 2^k complex row-major
+ * (2 * 4^k doubles).  The apply step for the fused blocks of the fusion pre-pass
+ * (python/gate_fusion.py:15-37 emits only 2x2; k > 1 blocks are new). */
+int llsv_apply_dense(llsv_state* s, int k, const int* targets, const double* m);
+
+/* One op of a gate program (a whole circuit handed over at once). */
+typedef enum {
+    LLSV_OP_1Q = 0,    /* q[0] = target, m[8] = the 2x2 gate */
+    LLSV_OP_CNOT = 1,  /* q[0] = control, q[1] = target */
+    LLSV_OP_CZ = 2,    /* q[0], q[1] */
+    LLSV_OP_MCZ = 3,   /* q[0..nq-1] */
+    LLSV_OP_DENSE = 4  /* q[0..nq-1] = targets, dense = 2 * 4^nq doubles */
+} llsv_op_kind;
+
+typedef struct {
+    int32_t kind;      /* llsv_op_kind */
+    int32_t nq;        /* qubits used in q[] */
+    int32_t q[8];
+    double m[8];       /* LLSV_OP_1Q matrix */
+    const double* dense; /* LLSV_OP_DENSE matrix (host memory), else NULL */
+} llsv_op;
+
+/* Apply ops[0..n_ops) in order.  The engine groups ops into HBM passes: gates whose
+ * qubits fit one on-chip tile are applied together in LDS during a single read and
+ * write of the state.  Results equal op-by-op application up to rounding.
+ * Replaces the per-gate Python loop of benchmarks/benchmark_fusion.py:49-50. */
+int llsv_apply_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops);
+
+/* ---- readout ---------------------------------------------------------------- */
+
+/* sum_i |psi_i|^2 (fixed-order reduction: bitwise reproducible run to run). */
+int llsv_norm2(llsv_state* s, double* out);
+
+/* <Z_q> = P(bit q = 0) - P(bit q = 1). */
+int llsv_expect_z(llsv_state* s, int q, double* out);
+
+/* |psi_i|^2 into host out[0..n) (n = 2^n_qubits).
+ * Replaces StateVectorBackend::probabilities, src/StateVectorBackend.h:153-161. */
+int llsv_probabilities(llsv_state* s, double* out, uint64_t n);
+
+/* Copies of the amplitudes (2 * n_amps doubles).  n_amps must equal the state size.
+ * copy_to_host replaces ll::Backend::state (src/Backend.h:44) and backs get_numpy_view
+ * (src/StateVectorOptimized.cpp:99-106). */
+int llsv_copy_to_host(llsv_state* s, double* amps, uint64_t n_amps);
+int llsv_copy_from_host(llsv_state* s, const double* amps, uint64_t n_amps);
+
+/* Wait for all queued work on the state's stream. */
+int llsv_sync(llsv_state* s);
+
+/* dst := src (same qubit count), device to device; ordered after work queued on both
+ * streams.  No reference counterpart: ll::Backend::state() (src/Backend.h:44) is the only
+ * way the reference copies a state, through the host. */
+int llsv_copy_state(llsv_state* dst, llsv_state* src);
+
+/* ---- interop and measurement ------------------------------------------------ */
+
+/* Device address of the amplitudes and the state's stream (hipStream_t as void*). */
+void* llsv_device_ptr(llsv_state* s);
+void* llsv_stream(llsv_state* s);
+
+/* HIP-event timer on the state's stream: start records an event, stop records a
+ * second one, waits for it and returns the elapsed milliseconds. */
+int llsv_timer_start(llsv_state* s);
+int llsv_timer_stop(llsv_state* s, float* ms);
+
+/* Per-kernel timing: llsv_mark records event `slot` (0 <= slot < LLSV_MARK_SLOTS) on the
+ * state's stream; llsv_mark_elapsed waits for event b and returns the milliseconds
+ * between events a and b. */
+#define LLSV_MARK_SLOTS 256
+int llsv_mark(llsv_state* s, int slot);
+int llsv_mark_elapsed(llsv_state* s, int a, int b, float* ms);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* llsv_last_error(void);
+
+/* Library version string. */
+const char* llsv_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LLSV_H */

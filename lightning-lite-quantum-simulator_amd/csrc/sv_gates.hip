@@ -1,0 +1,275 @@
+// sv_gates.hip -- streaming gfx950 kernels: one gate = one pass over the state in HBM.
+//
+// Every kernel here reads each amplitude it changes once and writes it once, with
+// 128-bit accesses; consecutive lanes of a 64-wide wavefront touch consecutive
+// amplitudes wherever the gate's index map allows, so each wavefront access is a
+// 1 KiB contiguous burst (runs of 2^q amplitudes when the gate's lowest qubit q < 6).
+// Grids are sized for >> 256 CUs; no data is reused, so the block -> tile order is
+// plain linear (XCD placement does not matter for a pure stream).
+#include "sv_common.h"
+#include "sv_kernels.h"
+
+namespace llsv {
+
+// ---------------------------------------------------------------- basis fill
+__global__ __launch_bounds__(256) void k_fill_basis(v2d* __restrict__ psi, uint64_t size, uint64_t index) {
+    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i < size) {
+        v2d v;
+        v.x = (i == index) ? 1.0 : 0.0;
+        v.y = 0.0;
+        st_nt(psi + i, v);
+    }
+}
+
+hipError_t launch_fill_basis(hipStream_t st, v2d* psi, int n, uint64_t index) {
+    const uint64_t size = uint64_t(1) << n;
+    const uint64_t blocks = (size + 255) / 256;
+    hipLaunchKernelGGL(k_fill_basis, dim3(unsigned(blocks)), dim3(256), 0, st, psi, size, index);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- copy
+// Plain device copy, 4 amplitudes per thread in flight.  Also the calibration kernel for
+// HBM traffic counters: it moves exactly 32 bytes per amplitude.
+__global__ __launch_bounds__(256) void k_copy(v2d* __restrict__ dst, const v2d* __restrict__ src, uint64_t size) {
+    const uint64_t i0 = uint64_t(blockIdx.x) * 1024 + threadIdx.x;
+    v2d v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        if (i0 + u * 256 < size) v[u] = ld_nt(src + i0 + u * 256);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+        if (i0 + u * 256 < size) st_nt(dst + i0 + u * 256, v[u]);
+}
+
+hipError_t launch_copy(hipStream_t st, v2d* dst, const v2d* src, int n) {
+    const uint64_t size = uint64_t(1) << n;
+    hipLaunchKernelGGL(k_copy, dim3(unsigned((size + 1023) / 1024)), dim3(256), 0, st, dst, src, size);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- dense 2x2
+// Pair p (bit q removed) -> i0 = insert0(p, q), i1 = i0 + 2^q.  Each thread owns PPT pairs
+// spaced one block apart, so a wavefront still covers 64 consecutive pairs per access
+// while every thread keeps 2 * PPT independent 16-byte loads in flight.
+template <int PPT>
+__global__ __launch_bounds__(256) void k_apply_1q(v2d* __restrict__ psi, uint64_t npairs, int q, Gate2 g) {
+    const uint64_t lomask = (uint64_t(1) << q) - 1;
+    const uint64_t stride = uint64_t(1) << q;
+    const uint64_t p0 = uint64_t(blockIdx.x) * (256 * PPT) + threadIdx.x;
+    v2d a0[PPT], a1[PPT];
+    uint64_t i0[PPT];
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        const uint64_t p = p0 + uint64_t(u) * 256;
+        i0[u] = ((p & ~lomask) << 1) | (p & lomask);
+        if (p < npairs) {
+            a0[u] = ld_nt(psi + i0[u]);
+            a1[u] = ld_nt(psi + i0[u] + stride);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        const uint64_t p = p0 + uint64_t(u) * 256;
+        if (p < npairs) {
+            st_nt(psi + i0[u], row2(g.r, a0[u], a1[u]));
+            st_nt(psi + i0[u] + stride, row2(g.r + 4, a0[u], a1[u]));
+        }
+    }
+}
+
+// Diagonal gate diag(1, d1): only the bit-q = 1 half changes (Z, S, T, phase gates).
+template <int PPT>
+__global__ __launch_bounds__(256) void k_phase_upper(v2d* __restrict__ psi, uint64_t npairs, int q, double dr,
+                                                     double di) {
+    const uint64_t lomask = (uint64_t(1) << q) - 1;
+    const uint64_t stride = uint64_t(1) << q;
+    const uint64_t p0 = uint64_t(blockIdx.x) * (256 * PPT) + threadIdx.x;
+    v2d a[PPT];
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        const uint64_t p = p0 + uint64_t(u) * 256;
+        if (p < npairs) a[u] = ld_nt(psi + (((p & ~lomask) << 1) | (p & lomask)) + stride);
+    }
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        const uint64_t p = p0 + uint64_t(u) * 256;
+        if (p < npairs) st_nt(psi + (((p & ~lomask) << 1) | (p & lomask)) + stride, cmul(dr, di, a[u]));
+    }
+}
+
+// Diagonal gate diag(d0, d1) with d0 != 1: every amplitude is scaled, one linear stream.
+template <int PPT>
+__global__ __launch_bounds__(256) void k_diag_1q(v2d* __restrict__ psi, uint64_t size, int q, Gate2 g) {
+    const uint64_t i0 = uint64_t(blockIdx.x) * (256 * PPT) + threadIdx.x;
+    v2d a[PPT];
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        const uint64_t i = i0 + uint64_t(u) * 256;
+        if (i < size) a[u] = ld_nt(psi + i);
+    }
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        const uint64_t i = i0 + uint64_t(u) * 256;
+        if (i < size) {
+            const bool hi = (i >> q) & 1;
+            st_nt(psi + i, hi ? cmul(g.r[6], g.r[7], a[u]) : cmul(g.r[0], g.r[1], a[u]));
+        }
+    }
+}
+
+constexpr int kPPT = 4;
+
+hipError_t launch_apply_1q(hipStream_t st, v2d* psi, int n, int q, const Gate2& g) {
+    const uint64_t npairs = uint64_t(1) << (n - 1);
+    const bool diag = g.r[2] == 0.0 && g.r[3] == 0.0 && g.r[4] == 0.0 && g.r[5] == 0.0;
+    if (diag && g.r[0] == 1.0 && g.r[1] == 0.0) {
+        if (g.r[6] == 1.0 && g.r[7] == 0.0) return hipSuccess;  // identity
+        const uint64_t blocks = (npairs + 256 * kPPT - 1) / (256 * kPPT);
+        hipLaunchKernelGGL(k_phase_upper<kPPT>, dim3(unsigned(blocks)), dim3(256), 0, st, psi, npairs, q, g.r[6],
+                           g.r[7]);
+    } else if (diag) {
+        const uint64_t size = uint64_t(1) << n;
+        const uint64_t blocks = (size + 256 * kPPT - 1) / (256 * kPPT);
+        hipLaunchKernelGGL(k_diag_1q<kPPT>, dim3(unsigned(blocks)), dim3(256), 0, st, psi, size, q, g);
+    } else {
+        const uint64_t blocks = (npairs + 256 * kPPT - 1) / (256 * kPPT);
+        hipLaunchKernelGGL(k_apply_1q<kPPT>, dim3(unsigned(blocks)), dim3(256), 0, st, psi, npairs, q, g);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- CNOT
+// 2^(n-2) swaps: index p with the two gate bits removed; i = insert(p) | control bit,
+// partner i | target bit.  Runs of 2^min(c, t) consecutive amplitudes per wavefront access.
+template <int PPT>
+__global__ __launch_bounds__(256) void k_cnot(v2d* __restrict__ psi, uint64_t nswaps, int lo, int hi, uint64_t cb,
+                                              uint64_t tb) {
+    const uint64_t p0 = uint64_t(blockIdx.x) * (256 * PPT) + threadIdx.x;
+    v2d a[PPT], b[PPT];
+    uint64_t idx[PPT];
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        const uint64_t p = p0 + uint64_t(u) * 256;
+        idx[u] = insert0(insert0(p, lo), hi) | cb;
+        if (p < nswaps) {
+            a[u] = ld_nt(psi + idx[u]);
+            b[u] = ld_nt(psi + (idx[u] | tb));
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        const uint64_t p = p0 + uint64_t(u) * 256;
+        if (p < nswaps) {
+            st_nt(psi + idx[u], b[u]);
+            st_nt(psi + (idx[u] | tb), a[u]);
+        }
+    }
+}
+
+hipError_t launch_cnot(hipStream_t st, v2d* psi, int n, int c, int t) {
+    const uint64_t nswaps = uint64_t(1) << (n - 2);
+    const int lo = c < t ? c : t, hi = c < t ? t : c;
+    const uint64_t blocks = (nswaps + 256 * kPPT - 1) / (256 * kPPT);
+    hipLaunchKernelGGL(k_cnot<kPPT>, dim3(unsigned(blocks)), dim3(256), 0, st, psi, nswaps, lo, hi,
+                       uint64_t(1) << c, uint64_t(1) << t);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- MCZ / CZ
+// Only the 2^(n-m) amplitudes with all m listed bits set are touched: index p with the
+// m bits removed, bits re-inserted in ascending position order, then OR the mask.
+template <int PPT>
+__global__ __launch_bounds__(256) void k_mcz(v2d* __restrict__ psi, uint64_t count, BitList bits) {
+    const uint64_t p0 = uint64_t(blockIdx.x) * (256 * PPT) + threadIdx.x;
+    v2d a[PPT];
+    uint64_t idx[PPT];
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        const uint64_t p = p0 + uint64_t(u) * 256;
+        uint64_t i = p;
+        for (int j = 0; j < bits.m; ++j) i = insert0(i, bits.pos[j]);
+        idx[u] = i | bits.mask;
+        if (p < count) a[u] = ld_nt(psi + idx[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        const uint64_t p = p0 + uint64_t(u) * 256;
+        if (p < count) st_nt(psi + idx[u], -a[u]);
+    }
+}
+
+hipError_t launch_mcz(hipStream_t st, v2d* psi, int n, const BitList& bits) {
+    const uint64_t count = uint64_t(1) << (n - bits.m);
+    const uint64_t blocks = (count + 256 * kPPT - 1) / (256 * kPPT);
+    hipLaunchKernelGGL(k_mcz<kPPT>, dim3(unsigned(blocks)), dim3(256), 0, st, psi, count, bits);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- dense k-qubit (VALU)
+// Block of 256 threads = 256 / D groups of D = 2^k amplitudes.  Thread (g, r) gathers
+// amplitude r of its group into LDS, then computes row r: y = sum_c U[r][c] x[c] with U
+// staged once per block in LDS.  Each block walks `iters` consecutive group batches.
+template <int K>
+__global__ __launch_bounds__(256) void k_dense(v2d* __restrict__ psi, uint64_t ngroups, BitList tb,
+                                               const v2d* __restrict__ U, const uint64_t* __restrict__ off,
+                                               int iters) {
+    constexpr int D = 1 << K;
+    constexpr int GPB = 256 / D;
+    __shared__ v2d sU[D * D];
+    __shared__ v2d sx[256];
+    for (int i = threadIdx.x; i < D * D; i += 256) sU[i] = U[i];
+    const int r = threadIdx.x % D;
+    const int gl = threadIdx.x / D;
+    const uint64_t my_off = off[r];
+    __syncthreads();
+    for (int it = 0; it < iters; ++it) {
+        const uint64_t g = (uint64_t(blockIdx.x) * iters + it) * GPB + gl;
+        uint64_t base = g;
+        for (int j = 0; j < tb.m; ++j) base = insert0(base, tb.pos[j]);
+        const bool live = g < ngroups;
+        if (live) sx[threadIdx.x] = ld_nt(psi + (base | my_off));
+        __syncthreads();
+        if (live) {
+            v2d acc = {0.0, 0.0};
+            const v2d* xr = sx + gl * D;
+            const v2d* ur = sU + r * D;
+#pragma unroll 8
+            for (int c = 0; c < D; ++c) {
+                const v2d u = ur[c], x = xr[c];
+                acc.x += u.x * x.x - u.y * x.y;
+                acc.y += u.x * x.y + u.y * x.x;
+            }
+            st_nt(psi + (base | my_off), acc);
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_dense(hipStream_t st, v2d* psi, int n, int k, const BitList& sorted_targets, const v2d* U_dev,
+                        const uint64_t* off_dev) {
+    const uint64_t ngroups = uint64_t(1) << (n - k);
+    const int gpb = 256 >> k;
+    const uint64_t batches = (ngroups + gpb - 1) / gpb;
+    const int iters = batches >= 4096 ? 8 : 1;
+    const uint64_t blocks = (batches + iters - 1) / iters;
+#define LLSV_DENSE_CASE(KK)                                                                                \
+    case KK:                                                                                               \
+        hipLaunchKernelGGL(k_dense<KK>, dim3(unsigned(blocks)), dim3(256), 0, st, psi, ngroups, sorted_targets, \
+                           U_dev, off_dev, iters);                                                         \
+        break;
+    switch (k) {
+        LLSV_DENSE_CASE(1)
+        LLSV_DENSE_CASE(2)
+        LLSV_DENSE_CASE(3)
+        LLSV_DENSE_CASE(4)
+        LLSV_DENSE_CASE(5)
+        default:
+            return hipErrorInvalidValue;
+    }
+#undef LLSV_DENSE_CASE
+    return hipGetLastError();
+}
+
+}  // namespace llsv

@@ -1,0 +1,63 @@
+"""The fusion pre-pass: GateFuser parity with python/gate_fusion.py and fuse_dense equivalence."""
+import numpy as np
+import pytest
+
+import gate_fusion as GF
+from helpers import random_ops
+from oracle import oracle as O
+
+
+def test_multiply_order():
+    # python/gate_fusion.py:11: result = g2 @ g1 ("g1 first")
+    h, s = GF.GATES["H"], GF.GATES["S"]
+    got = np.array(GF.GateFuser.multiply_gates(h, s)).reshape(2, 2)
+    assert np.allclose(got, np.array(s).reshape(2, 2) @ np.array(h).reshape(2, 2), atol=0)
+    assert isinstance(GF.GateFuser.multiply_gates(h, s), list)
+
+
+def test_fuse_circuit_benchmark_shape():
+    # benchmarks/benchmark_fusion.py:14-27: 10 x 20 qubits x (H, X, S, T) = 800 -> 200
+    circuit = [(q, GF.GATES[g]) for _ in range(10) for q in range(20) for g in "HXST"]
+    fused = GF.GateFuser.fuse_circuit(circuit)
+    assert len(circuit) == 800 and len(fused) == 200
+    a = O.run(20 if False else 5, [])  # keep oracle import warm
+    st1 = O.run(5, [("u", q, g) for q, g in circuit if q < 5])
+    st2 = O.run(5, [("u", q, g) for q, g in fused if q < 5])
+    assert np.max(np.abs(st1 - st2)) < 1e-13
+
+
+def test_fuse_circuit_only_adjacent():
+    c = [(0, GF.GATES["H"]), (1, GF.GATES["X"]), (0, GF.GATES["H"])]
+    assert len(GF.GateFuser.fuse_circuit(c)) == 3
+    assert GF.GateFuser.fuse_circuit([]) == []
+
+
+def test_analyze_circuit(capsys):
+    out = GF.GateFuser.analyze_circuit([(0, GF.GATES["H"]), (0, GF.GATES["X"])])
+    assert len(out) == 1 and "Reduction: 50.0%" in capsys.readouterr().out
+
+
+@pytest.mark.parametrize("max_k", [1, 2, 3, 4, 5])
+def test_fuse_dense_equivalent(max_k):
+    rng = np.random.default_rng(40 + max_k)
+    n = 7
+    ops = random_ops(rng, n, 120, max_k=min(max_k, 3))
+    fused = GF.fuse_dense(ops, max_k)
+    assert len(fused) <= len(ops)
+    for op in fused:
+        if op[0] == "dense":
+            k = len(op[1])
+            assert k <= max_k
+            U = np.asarray(op[2])
+            assert np.max(np.abs(U.conj().T @ U - np.eye(1 << k))) < 1e-12
+    assert np.max(np.abs(O.run(n, fused) - O.run(n, ops))) < 1e-12
+    assert np.max(np.abs(O.run_bruteforce(n, fused) - O.run(n, ops))) < 1e-12
+
+
+def test_fuse_dense_layered_block_count():
+    from llsv import circuits as C
+
+    ops = C.layered_rotations(1, 10, 2)
+    fused = GF.fuse_dense(ops, 5)
+    assert len(fused) < len(ops) // 4
+    assert np.max(np.abs(O.run(10, fused) - O.run(10, ops))) < 1e-12
