@@ -110,6 +110,17 @@ typedef struct {
  * Replaces the per-gate Python loop of benchmarks/benchmark_fusion.py:49-50. */
 int llsv_apply_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops);
 
+/* The pass plan llsv_apply_program would run for this circuit with tiles of 2^tile_bits
+ * amplitudes (host only, no device needed): *n_passes gets the number of HBM passes;
+ * order[0..n_ops), if not null, the op indices in execution order; pass_end[p], if not
+ * null, the end of pass p in order[] (size n_ops is always enough).  Ops are only moved
+ * past ops they share no qubit with. */
+int llsv_plan_passes(int n_qubits, const llsv_op* ops, uint64_t n_ops, int tile_bits, uint64_t* n_passes,
+                     uint64_t* order, uint64_t* pass_end);
+
+/* Tile bits llsv_apply_program uses on this state's device (from the LDS per workgroup). */
+int llsv_program_tile_bits(llsv_state* s);
+
 /* ---- readout ---------------------------------------------------------------- */
 
 /* sum_i |psi_i|^2 (fixed-order reduction: bitwise reproducible run to run). */

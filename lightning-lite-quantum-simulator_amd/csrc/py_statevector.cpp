@@ -73,6 +73,7 @@ public:
         const uint64_t size = static_cast<uint64_t>(a.shape(0));
         if (size < 2 || (size & (size - 1)) != 0) throw std::runtime_error("Array size must be power of 2");
         borrowed_ = a;
+        is_borrowed_ = true;
         host_ = reinterpret_cast<double*>(a.mutable_data());
         int rc;
         {
@@ -176,7 +177,7 @@ public:
         if (anchor.is_none()) {
             anchor = py::cast(ViewAnchor{self});
             anchor_ = py::weakref(anchor);
-            if (!borrowed_) pull();  // a fresh view of an owned state starts from the device copy
+            if (!is_borrowed_) pull();  // a fresh view of an owned state starts from the device copy
         }
         return py::array(py::dtype::of<cplx>(), {static_cast<py::ssize_t>(size())},
                          {static_cast<py::ssize_t>(sizeof(cplx))}, host_, anchor);
@@ -199,6 +200,11 @@ public:
         device_call([&] { return llsv_timer_stop(st_, &ms); });
         return ms;
     }
+    int tile_bits() {
+        const int b = llsv_program_tile_bits(st_);
+        if (b < 0) raise_rc(b);
+        return b;
+    }
     void mark(int slot) { device_call([&] { return llsv_mark(st_, slot); }); }
     float mark_elapsed(int a, int b) {
         float ms = 0.f;
@@ -213,7 +219,8 @@ public:
 
 private:
     llsv_state* st_ = nullptr;
-    py::array borrowed_;
+    py::array borrowed_;  // keeps the borrowed array alive (a default py::array is an empty array, not null)
+    bool is_borrowed_ = false;
     py::array_t<cplx> mirror_;
     double* host_ = nullptr;
     py::object anchor_ = py::none();
@@ -227,7 +234,7 @@ private:
 
     bool host_linked() {
         if (!host_) return false;
-        if (borrowed_) return true;
+        if (is_borrowed_) return true;
         return !anchor_alive().is_none();
     }
 
@@ -361,6 +368,7 @@ PYBIND11_MODULE(LLSV_PYMOD, m) {
         .def("host_coherent", &StateVector::host_coherent)
         .def("timer_start", &StateVector::timer_start)
         .def("timer_stop", &StateVector::timer_stop)
+        .def("program_tile_bits", &StateVector::tile_bits)
         .def("mark", &StateVector::mark, py::arg("slot"))
         .def("mark_elapsed", &StateVector::mark_elapsed, py::arg("a"), py::arg("b"))
         .def("copy_from", &StateVector::copy_from, py::arg("other"))

@@ -19,21 +19,7 @@
 #include "sv_kernels.h"
 #include "sv_program.h"
 
-struct llsv_state {
-    int n = 0;
-    uint64_t size = 0;
-    int device = 0;
-    llsv::v2d* amps = nullptr;
-    hipStream_t stream = nullptr;
-    double* partials = nullptr;      // 2 * kRedBlocks doubles
-    double* result_dev = nullptr;    // 4 doubles
-    double* result_host = nullptr;   // 4 doubles, pinned
-    unsigned char* params = nullptr; // device parameter area
-    size_t params_bytes = 0;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipEvent_t marks[LLSV_MARK_SLOTS] = {};
-    llsv::ProgramCache* program = nullptr;
-};
+#include "sv_state.h"
 
 namespace llsv {
 
@@ -41,7 +27,7 @@ static thread_local std::string g_last_error;
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 
-static int fail(int code, const char* fmt, ...) {
+int fail(int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -376,38 +362,3 @@ int llsv_timer_stop(llsv_state* s, float* ms) {
 
 }  // extern "C"
 
-// ---------------------------------------------------------------- program (internal)
-namespace llsv {
-
-int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
-    for (uint64_t i = 0; i < n_ops; ++i) {
-        const llsv_op& op = ops[i];
-        int rc = LLSV_OK;
-        switch (op.kind) {
-            case LLSV_OP_1Q:
-                rc = llsv_apply_1q(s, op.q[0], op.m);
-                break;
-            case LLSV_OP_CNOT:
-                rc = llsv_apply_cnot(s, op.q[0], op.q[1]);
-                break;
-            case LLSV_OP_CZ:
-                rc = llsv_apply_cz(s, op.q[0], op.q[1]);
-                break;
-            case LLSV_OP_MCZ:
-                rc = (op.nq < 1 || op.nq > 8) ? fail(LLSV_ERR_ARG, "llsv_apply_program: op %llu: bad qubit count",
-                                                     (unsigned long long)i)
-                                              : llsv_apply_mcz(s, op.q, op.nq);
-                break;
-            case LLSV_OP_DENSE:
-                rc = llsv_apply_dense(s, op.nq, op.q, op.dense);
-                break;
-            default:
-                rc = fail(LLSV_ERR_ARG, "llsv_apply_program: op %llu: unknown kind %d", (unsigned long long)i,
-                          op.kind);
-        }
-        if (rc != LLSV_OK) return rc;
-    }
-    return LLSV_OK;
-}
-
-}  // namespace llsv

@@ -44,5 +44,7 @@ __device__ __forceinline__ void st_nt(v2d* p, v2d v) { __builtin_nontemporal_sto
 
 // Thread-local error message for llsv_last_error().
 void set_error(const std::string& msg);
+// Formats the message, stores it for llsv_last_error() and returns code.
+int fail(int code, const char* fmt, ...);
 
 }  // namespace llsv
