@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="launch each kernel a few times and exit (for rocprofv3 counter runs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {a.gpus} is not supported (the state-vector "
@@ -84,6 +86,23 @@ def main():
 
     log(f"allocating {n}-qubit state ({16 * size / 2**30:.1f} GiB)")
     sv = V3.StateVector(n)
+
+    if a.probe:
+        # 2 copies (calibration: 32 B per amplitude), 3 one-qubit gates (low / mid / high
+        # target), 2 CNOTs and one program step, each a separate dispatch.
+        other = V3.StateVector(n)
+        for _ in range(2):
+            other.copy_from(sv)
+        other.sync()
+        del other
+        for q in (0, n // 2, n - 1):
+            sv.apply_gate_cache_optimized(q, one_q[q][1])
+        sv.cnot(0, 1)
+        sv.cnot(n - 2, n - 1)
+        sv.apply_program(ops)
+        sv.sync()
+        print(json.dumps({"probe": n, "norm": sv.norm()}), flush=True)
+        return
 
     def step(marks=None):
         slot = 0
