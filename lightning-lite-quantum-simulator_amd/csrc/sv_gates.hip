@@ -79,6 +79,30 @@ __global__ __launch_bounds__(256) void k_apply_1q(v2d* __restrict__ psi, uint64_
     }
 }
 
+// Target q < 6: the partner of amplitude i (i ^ 2^q) sits in the same wavefront when each
+// lane loads one of 64 consecutive amplitudes, so it is fetched with a cross-lane shuffle and
+// every load and store instruction covers 1 KiB of contiguous memory.  (Pairing by index
+// instead would make each store write every other 16 bytes of a line, which costs up to 66%
+// extra HBM write traffic for q <= 2.)
+template <int PPT>
+__global__ __launch_bounds__(256) void k_apply_1q_lane(v2d* __restrict__ psi, uint64_t size, int q, Gate2 g) {
+    const uint64_t i0 = uint64_t(blockIdx.x) * (256 * PPT) + threadIdx.x;
+    const int m = 1 << q;
+    const bool hi = (threadIdx.x >> q) & 1;
+    const double* r = hi ? g.r + 4 : g.r;
+    v2d a[PPT];
+#pragma unroll
+    for (int u = 0; u < PPT; ++u)
+        if (i0 + u * 256 < size) a[u] = ld_nt(psi + i0 + u * 256);
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        v2d p;
+        p.x = __shfl_xor(a[u].x, m, 64);
+        p.y = __shfl_xor(a[u].y, m, 64);
+        if (i0 + u * 256 < size) st_nt(psi + i0 + u * 256, hi ? row2(r, p, a[u]) : row2(r, a[u], p));
+    }
+}
+
 // Diagonal gate diag(1, d1): only the bit-q = 1 half changes (Z, S, T, phase gates).
 template <int PPT>
 __global__ __launch_bounds__(256) void k_phase_upper(v2d* __restrict__ psi, uint64_t npairs, int q, double dr,
@@ -120,11 +144,13 @@ __global__ __launch_bounds__(256) void k_diag_1q(v2d* __restrict__ psi, uint64_t
 }
 
 constexpr int kPPT = 4;
+// Lowest qubit whose 2^q-amplitude runs (16 * 2^q bytes) fill whole 128-byte lines.
+constexpr int kLineBits = 3;
 
 hipError_t launch_apply_1q(hipStream_t st, v2d* psi, int n, int q, const Gate2& g) {
     const uint64_t npairs = uint64_t(1) << (n - 1);
     const bool diag = g.r[2] == 0.0 && g.r[3] == 0.0 && g.r[4] == 0.0 && g.r[5] == 0.0;
-    if (diag && g.r[0] == 1.0 && g.r[1] == 0.0) {
+    if (diag && g.r[0] == 1.0 && g.r[1] == 0.0 && q >= kLineBits) {
         if (g.r[6] == 1.0 && g.r[7] == 0.0) return hipSuccess;  // identity
         const uint64_t blocks = (npairs + 256 * kPPT - 1) / (256 * kPPT);
         hipLaunchKernelGGL(k_phase_upper<kPPT>, dim3(unsigned(blocks)), dim3(256), 0, st, psi, npairs, q, g.r[6],
@@ -133,17 +159,27 @@ hipError_t launch_apply_1q(hipStream_t st, v2d* psi, int n, int q, const Gate2& 
         const uint64_t size = uint64_t(1) << n;
         const uint64_t blocks = (size + 256 * kPPT - 1) / (256 * kPPT);
         hipLaunchKernelGGL(k_diag_1q<kPPT>, dim3(unsigned(blocks)), dim3(256), 0, st, psi, size, q, g);
+    } else if (q < 6 && n >= 6) {
+        const uint64_t size = uint64_t(1) << n;
+        const uint64_t blocks = (size + 256 * kPPT - 1) / (256 * kPPT);
+        hipLaunchKernelGGL(k_apply_1q_lane<kPPT>, dim3(unsigned(blocks)), dim3(256), 0, st, psi, size, q, g);
+    } else if (q >= 11) {
+        const uint64_t blocks = (npairs + 256 * 4 - 1) / (256 * 4);
+        hipLaunchKernelGGL(k_apply_1q<4>, dim3(unsigned(blocks)), dim3(256), 0, st, psi, npairs, q, g);
     } else {
-        const uint64_t blocks = (npairs + 256 * kPPT - 1) / (256 * kPPT);
-        hipLaunchKernelGGL(k_apply_1q<kPPT>, dim3(unsigned(blocks)), dim3(256), 0, st, psi, npairs, q, g);
+        const uint64_t blocks = (npairs + 256 * 2 - 1) / (256 * 2);
+        hipLaunchKernelGGL(k_apply_1q<2>, dim3(unsigned(blocks)), dim3(256), 0, st, psi, npairs, q, g);
     }
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- CNOT
-// 2^(n-2) swaps: index p with the two gate bits removed; i = insert(p) | control bit,
-// partner i | target bit.  Runs of 2^min(c, t) consecutive amplitudes per wavefront access.
-template <int PPT>
+// Control c >= 3: only the control = 1 half moves, 2^(n-2) swaps: index p with the two gate
+// bits removed; i = insert(p) | control bit, partner i | target bit.  Every touched run is
+// >= 8 amplitudes (a whole 128-byte line).  Non-temporal accesses when the target is >= 3
+// too; with a low target the two halves of each swap share lines, and cached stores let L2
+// merge them.
+template <int PPT, bool NT>
 __global__ __launch_bounds__(256) void k_cnot(v2d* __restrict__ psi, uint64_t nswaps, int lo, int hi, uint64_t cb,
                                               uint64_t tb) {
     const uint64_t p0 = uint64_t(blockIdx.x) * (256 * PPT) + threadIdx.x;
@@ -154,26 +190,93 @@ __global__ __launch_bounds__(256) void k_cnot(v2d* __restrict__ psi, uint64_t ns
         const uint64_t p = p0 + uint64_t(u) * 256;
         idx[u] = insert0(insert0(p, lo), hi) | cb;
         if (p < nswaps) {
-            a[u] = ld_nt(psi + idx[u]);
-            b[u] = ld_nt(psi + (idx[u] | tb));
+            a[u] = NT ? ld_nt(psi + idx[u]) : psi[idx[u]];
+            b[u] = NT ? ld_nt(psi + (idx[u] | tb)) : psi[idx[u] | tb];
         }
     }
 #pragma unroll
     for (int u = 0; u < PPT; ++u) {
         const uint64_t p = p0 + uint64_t(u) * 256;
         if (p < nswaps) {
-            st_nt(psi + idx[u], b[u]);
-            st_nt(psi + (idx[u] | tb), a[u]);
+            if (NT) {
+                st_nt(psi + idx[u], b[u]);
+                st_nt(psi + (idx[u] | tb), a[u]);
+            } else {
+                psi[idx[u]] = b[u];
+                psi[idx[u] | tb] = a[u];
+            }
+        }
+    }
+}
+
+// Control c < 3: the control = 1 amplitudes are interleaved with the others at 16-byte
+// granularity, so every line is touched anyway.  One full pass over whole lines instead:
+// target t < 6 pairs through a cross-lane shuffle, t >= 6 through the index pairing of the
+// one-qubit kernel; amplitudes whose control bit is 0 are written back unchanged.
+template <int PPT>
+__global__ __launch_bounds__(256) void k_cnot_lane(v2d* __restrict__ psi, uint64_t size, int c, int t) {
+    const uint64_t i0 = uint64_t(blockIdx.x) * (256 * PPT) + threadIdx.x;
+    const int m = 1 << t;
+    v2d a[PPT];
+#pragma unroll
+    for (int u = 0; u < PPT; ++u)
+        if (i0 + u * 256 < size) a[u] = ld_nt(psi + i0 + u * 256);
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        v2d p;
+        p.x = __shfl_xor(a[u].x, m, 64);
+        p.y = __shfl_xor(a[u].y, m, 64);
+        const uint64_t i = i0 + u * 256;
+        if (i < size) st_nt(psi + i, ((i >> c) & 1) ? p : a[u]);
+    }
+}
+
+template <int PPT>
+__global__ __launch_bounds__(256) void k_cnot_pairs(v2d* __restrict__ psi, uint64_t npairs, int c, int t) {
+    const uint64_t lomask = (uint64_t(1) << t) - 1;
+    const uint64_t stride = uint64_t(1) << t;
+    const uint64_t p0 = uint64_t(blockIdx.x) * (256 * PPT) + threadIdx.x;
+    v2d a0[PPT], a1[PPT];
+    uint64_t i0[PPT];
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        const uint64_t p = p0 + uint64_t(u) * 256;
+        i0[u] = ((p & ~lomask) << 1) | (p & lomask);
+        if (p < npairs) {
+            a0[u] = ld_nt(psi + i0[u]);
+            a1[u] = ld_nt(psi + i0[u] + stride);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        const uint64_t p = p0 + uint64_t(u) * 256;
+        if (p < npairs) {
+            const bool sw = (i0[u] >> c) & 1;
+            st_nt(psi + i0[u], sw ? a1[u] : a0[u]);
+            st_nt(psi + i0[u] + stride, sw ? a0[u] : a1[u]);
         }
     }
 }
 
 hipError_t launch_cnot(hipStream_t st, v2d* psi, int n, int c, int t) {
-    const uint64_t nswaps = uint64_t(1) << (n - 2);
     const int lo = c < t ? c : t, hi = c < t ? t : c;
-    const uint64_t blocks = (nswaps + 256 * kPPT - 1) / (256 * kPPT);
-    hipLaunchKernelGGL(k_cnot<kPPT>, dim3(unsigned(blocks)), dim3(256), 0, st, psi, nswaps, lo, hi,
-                       uint64_t(1) << c, uint64_t(1) << t);
+    if (c >= kLineBits || n < 6) {
+        const uint64_t nswaps = uint64_t(1) << (n - 2);
+        const uint64_t blocks = (nswaps + 256 * kPPT - 1) / (256 * kPPT);
+        if (t >= kLineBits)
+            hipLaunchKernelGGL((k_cnot<kPPT, true>), dim3(unsigned(blocks)), dim3(256), 0, st, psi, nswaps, lo, hi,
+                               uint64_t(1) << c, uint64_t(1) << t);
+        else
+            hipLaunchKernelGGL((k_cnot<kPPT, false>), dim3(unsigned(blocks)), dim3(256), 0, st, psi, nswaps, lo, hi,
+                               uint64_t(1) << c, uint64_t(1) << t);
+    } else if (t < 6) {
+        const uint64_t size = uint64_t(1) << n;
+        hipLaunchKernelGGL(k_cnot_lane<kPPT>, dim3(unsigned((size + 1023) / 1024)), dim3(256), 0, st, psi, size, c, t);
+    } else {
+        const uint64_t npairs = uint64_t(1) << (n - 1);
+        hipLaunchKernelGGL(k_cnot_pairs<kPPT>, dim3(unsigned((npairs + 1023) / 1024)), dim3(256), 0, st, psi, npairs,
+                           c, t);
+    }
     return hipGetLastError();
 }
 
@@ -200,7 +303,29 @@ __global__ __launch_bounds__(256) void k_mcz(v2d* __restrict__ psi, uint64_t cou
     }
 }
 
+// Lowest masked qubit < 3: the touched amplitudes are spread over every line, so stream
+// the whole state once (negating the matching ones) rather than write partial lines.
+template <int PPT>
+__global__ __launch_bounds__(256) void k_mcz_full(v2d* __restrict__ psi, uint64_t size, uint64_t mask) {
+    const uint64_t i0 = uint64_t(blockIdx.x) * (256 * PPT) + threadIdx.x;
+    v2d a[PPT];
+#pragma unroll
+    for (int u = 0; u < PPT; ++u)
+        if (i0 + u * 256 < size) a[u] = ld_nt(psi + i0 + u * 256);
+#pragma unroll
+    for (int u = 0; u < PPT; ++u) {
+        const uint64_t i = i0 + u * 256;
+        if (i < size) st_nt(psi + i, ((i & mask) == mask) ? -a[u] : a[u]);
+    }
+}
+
 hipError_t launch_mcz(hipStream_t st, v2d* psi, int n, const BitList& bits) {
+    if (bits.pos[0] < kLineBits && n >= 6) {
+        const uint64_t size = uint64_t(1) << n;
+        hipLaunchKernelGGL(k_mcz_full<kPPT>, dim3(unsigned((size + 1023) / 1024)), dim3(256), 0, st, psi, size,
+                           bits.mask);
+        return hipGetLastError();
+    }
     const uint64_t count = uint64_t(1) << (n - bits.m);
     const uint64_t blocks = (count + 256 * kPPT - 1) / (256 * kPPT);
     hipLaunchKernelGGL(k_mcz<kPPT>, dim3(unsigned(blocks)), dim3(256), 0, st, psi, count, bits);

@@ -112,8 +112,9 @@ def test_random_circuits_small(n):
     assert maxdiff(gpu_run(n, ops, program=True), ref) <= TOL
 
 
+@pytest.mark.parametrize("program", [False, True])
 @pytest.mark.parametrize("n", [18, 21])
-def test_every_target_and_pair(n):
+def test_every_target_and_pair(n, program):
     # every one-qubit gate kind on every target (dense, diagonal, phase-only paths),
     # CNOT/CZ over near and far pairs, dense blocks over low, high and mixed qubits
     rng = np.random.default_rng(n)
@@ -127,19 +128,32 @@ def test_every_target_and_pair(n):
     for targets in ([0], [n - 1], [0, 1], [n - 2, n - 1], [0, n - 1, 5], [1, 2, 3, 4], [n - 1, 0, 7, 3, 12]):
         ops.append(("dense", targets, random_unitary(rng, 1 << len(targets))))
     ops.append(("mcz", [0, 4, n - 1]))
-    say(f"every target n={n}")
+    ops += [("cz", 0, 1), ("cz", 1, n - 1), ("mcz", [2, 9, n - 2]), ("cnot", 2, 0), ("cnot", 1, 5), ("cnot", 0, n - 1)]
+    say(f"every target n={n} program={program}")
     ref = O.run(n, ops)
-    assert maxdiff(gpu_run(n, ops, program=True), ref) <= TOL
+    assert maxdiff(gpu_run(n, ops, program=program), ref) <= TOL
 
 
-def test_permutations_bit_exact():
+@pytest.mark.parametrize("program", [False, True])
+def test_permutations_bit_exact(program):
     rng = np.random.default_rng(8)
     n = 16
     start = (rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)).astype(np.complex128)
     ops = random_ops(rng, n, 300, kinds=("x", "cnot", "cz", "mcz"))
     sv = qsv().StateVector(start.copy())
     sv.defer_host_sync(True)
-    sv.apply_program(ops)
+    if program:
+        sv.apply_program(ops)
+    else:
+        for op in ops:
+            if op[0] == "x":
+                sv.apply_gate_cache_optimized(op[1], O.mat_x())
+            elif op[0] == "cnot":
+                sv.cnot(op[1], op[2])
+            elif op[0] == "cz":
+                sv.cz(op[1], op[2])
+            else:
+                sv.mcz(list(op[1]))
     got = sv.state()
     assert bits_equal(got, O.run(n, ops, psi=start))
 
