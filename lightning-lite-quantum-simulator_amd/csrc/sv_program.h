@@ -29,7 +29,15 @@ enum PassOpKind : int {
     P_CNOT = 3,    // a = local control, b = local target
     P_MCZ = 4,     // negate where (l & lmask) == lmask, if (tile base & omask) == omask
     P_DENSE = 5,   // a = k, q[0..k) local bits, moff = offset (in v2d) of the matrix
+    // register-resident variant (b = 12) only:
+    P_LAYOUT = 6,  // q[0..4) = the 4 local bits held in registers from now on (ascending)
+    R_1Q = 7,      // a = register bit (0..3), m = 2x2
+    R_CNOT = 8,    // a = local control bit, b = register bit of the target
 };
+
+// Register-resident tiles: 2^12 amplitudes per workgroup of 256 threads, 16 per thread.
+constexpr int kRegTileBits = 12;
+constexpr int kRegBits = 4;
 
 struct PassOp {
     int kind;

@@ -150,6 +150,180 @@ __global__ __launch_bounds__(NT) void k_pass(v2d* __restrict__ psi, PassDesc d, 
 
 constexpr int kPassThreads = 256;
 
+// ---------------------------------------------------------------- register-resident tiles
+// The tile (2^12 amplitudes) lives in registers, 16 per thread: 4 "register bits" S of the
+// local index select the register, the other 8 local bits are the thread id (ascending bit
+// order both ways).  A one-qubit gate or a CNOT whose target is in S is a pure register
+// update; diagonal ops (phases, CZ, MCZ) need no data movement in any layout, since each
+// thread knows the index of every amplitude it holds.  LDS is used only to change S (write
+// all, barrier, read all) and to feed dense blocks.  LDS addresses are XOR-swizzled
+// (bits 0..3 ^= bits 4..7) so that layouts whose thread bits start above bit 3 do not
+// serialise on one bank.
+__device__ __forceinline__ uint32_t swz(uint32_t l) { return l ^ ((l >> 4) & 15u); }
+
+__device__ __forceinline__ uint32_t reg_dep(int j, const uint32_t* sb) {
+    return ((j & 1) ? sb[0] : 0u) | ((j & 2) ? sb[1] : 0u) | ((j & 4) ? sb[2] : 0u) | ((j & 8) ? sb[3] : 0u);
+}
+
+__device__ __forceinline__ uint32_t thread_base(uint32_t tid, uint32_t smask) {
+    uint32_t out = 0;
+    int k = 0;
+#pragma unroll
+    for (int j = 0; j < kRegTileBits; ++j)
+        if (!((smask >> j) & 1u)) {
+            out |= ((tid >> k) & 1u) << j;
+            ++k;
+        }
+    return out;
+}
+
+__device__ __forceinline__ uint64_t local_to_global(const PassDesc& d, int bit) {
+    return bit < d.L ? (uint64_t(1) << bit) : (uint64_t(1) << d.tq[bit]);
+}
+
+template <int K>
+__device__ __forceinline__ void reg_1q(v2d* a, const double* m) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+        if (!(j & (1 << K))) {
+            const v2d x = a[j], y = a[j | (1 << K)];
+            a[j] = row2(m, x, y);
+            a[j | (1 << K)] = row2(m + 4, x, y);
+        }
+}
+
+template <int K>
+__device__ __forceinline__ void reg_cnot(v2d* a, uint32_t lt, const uint32_t* sb, uint32_t cbit) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+        if (!(j & (1 << K))) {
+            if ((lt | reg_dep(j, sb)) & cbit) {
+                const v2d x = a[j];
+                a[j] = a[j | (1 << K)];
+                a[j | (1 << K)] = x;
+            }
+        }
+}
+
+template <bool DENSE>
+__global__ __launch_bounds__(256) void k_pass_reg(v2d* __restrict__ psi, PassDesc d, const PassOp* __restrict__ ops,
+                                                  const v2d* __restrict__ mats) {
+    extern __shared__ v2d tile[];
+    const uint32_t tid = threadIdx.x;
+    uint64_t base = blockIdx.x;
+    for (int j = 0; j < kRegTileBits; ++j) base = insert0(base, d.pos[j]);
+
+    // initial layout: S = local bits 8..11, thread bits 0..7 (coalesced runs of 2^L)
+    uint32_t sb[4] = {1u << 8, 1u << 9, 1u << 10, 1u << 11};
+    uint32_t lt = tid;
+    v2d a[16];
+    {
+        uint64_t gt = 0;
+        for (int j = 0; j < 8; ++j)
+            if ((tid >> j) & 1u) gt |= local_to_global(d, j);
+        uint64_t gs[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) gs[i] = local_to_global(d, 8 + i);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const uint64_t go = ((j & 1) ? gs[0] : 0) | ((j & 2) ? gs[1] : 0) | ((j & 4) ? gs[2] : 0) | ((j & 8) ? gs[3] : 0);
+            a[j] = ld_nt(psi + (base | gt | go));
+        }
+    }
+
+    double sr = 1.0, si = 0.0;
+    for (int i = 0; i < d.nops; ++i) {
+        const PassOp& op = ops[i];
+        const int kind = op.kind;
+        if (kind == R_1Q) {
+            switch (op.a) {
+                case 0: reg_1q<0>(a, op.m); break;
+                case 1: reg_1q<1>(a, op.m); break;
+                case 2: reg_1q<2>(a, op.m); break;
+                default: reg_1q<3>(a, op.m); break;
+            }
+        } else if (kind == R_CNOT) {
+            const uint32_t cbit = 1u << op.a;
+            switch (op.b) {
+                case 0: reg_cnot<0>(a, lt, sb, cbit); break;
+                case 1: reg_cnot<1>(a, lt, sb, cbit); break;
+                case 2: reg_cnot<2>(a, lt, sb, cbit); break;
+                default: reg_cnot<3>(a, lt, sb, cbit); break;
+            }
+        } else if (kind == P_DIAG) {
+            const uint32_t bit = 1u << op.a;
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                a[j] = ((lt | reg_dep(j, sb)) & bit) ? cmul(op.m[6], op.m[7], a[j]) : cmul(op.m[0], op.m[1], a[j]);
+        } else if (kind == P_MCZ) {
+            if ((base & op.omask) == op.omask) {
+                const uint32_t lm = uint32_t(op.lmask);
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                    if (((lt | reg_dep(j, sb)) & lm) == lm) a[j] = -a[j];
+            }
+        } else if (kind == P_SCALE) {
+            const bool hi = (base >> op.a) & 1;
+            const double dr = hi ? op.m[6] : op.m[0], di = hi ? op.m[7] : op.m[1];
+            const double nr = sr * dr - si * di;
+            si = sr * di + si * dr;
+            sr = nr;
+        } else if (kind == P_LAYOUT) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) tile[swz(lt | reg_dep(j, sb))] = a[j];
+            __syncthreads();
+            uint32_t smask = 0;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                sb[t] = 1u << op.q[t];
+                smask |= sb[t];
+            }
+            lt = thread_base(tid, smask);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) a[j] = tile[swz(lt | reg_dep(j, sb))];
+            __syncthreads();
+        } else if (DENSE && kind == P_DENSE) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) tile[swz(lt | reg_dep(j, sb))] = a[j];
+            __syncthreads();
+            const int k = op.a, D = 1 << k;
+            uint32_t tmask = 0;
+            for (int t = 0; t < k; ++t) tmask |= 1u << op.q[t];
+            const v2d* U = mats + op.moff;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const uint32_t o = lt | reg_dep(j, sb);
+                uint32_t row = 0;
+                for (int t = 0; t < k; ++t) row |= ((o >> op.q[t]) & 1u) << t;
+                const uint32_t gb = o & ~tmask;
+                const v2d* ur = U + row * D;
+                v2d acc = {0.0, 0.0};
+                for (int c = 0; c < D; ++c) {
+                    const v2d u = ur[c], x = tile[swz(gb | op.dep[c])];
+                    acc.x += u.x * x.x - u.y * x.y;
+                    acc.y += u.x * x.y + u.y * x.x;
+                }
+                a[j] = acc;
+            }
+            __syncthreads();
+        }
+    }
+
+    // the planner ends every pass in a layout whose thread bits include local bits 0..L-1
+    uint64_t gt = 0;
+    for (int j = 0; j < kRegTileBits; ++j)
+        if ((lt >> j) & 1u) gt |= local_to_global(d, j);
+    uint64_t gs[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) gs[t] = local_to_global(d, 31 - __builtin_clz(sb[t]));
+    const bool unit = (sr == 1.0 && si == 0.0);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const uint64_t go = ((j & 1) ? gs[0] : 0) | ((j & 2) ? gs[1] : 0) | ((j & 4) ? gs[2] : 0) | ((j & 8) ? gs[3] : 0);
+        st_nt(psi + (base | gt | go), unit ? a[j] : cmul(sr, si, a[j]));
+    }
+}
+
 // ---------------------------------------------------------------- host: validation
 namespace {
 
@@ -307,6 +481,64 @@ int plan_order(int n, const llsv_op* ops, uint64_t n_ops, int b, int L, uint64_t
 
 static constexpr int kLowBits = 3;
 
+namespace {
+
+// Turns a pass's ops (tile-local coordinates) into the register-resident program: a layout
+// change is inserted before every one-qubit gate or CNOT whose target bit is not held in
+// registers; the new register set takes that bit, then the target bits of the next ops (so
+// one change serves several gates), then bits of the current set.  The pass ends in a layout
+// whose thread bits include local bits 0..L-1, so the stores write whole lines.
+std::vector<PassOp> to_register_program(const std::vector<PassOp>& in, int L) {
+    constexpr uint32_t kStart = 0xF00u;  // local bits 8..11
+    const uint32_t low = (1u << L) - 1;
+    auto need_of = [](const PassOp& po) -> uint32_t {
+        if (po.kind == P_1Q) return 1u << po.a;
+        if (po.kind == P_CNOT) return 1u << po.b;
+        return 0u;
+    };
+    auto rank = [](uint32_t set, int bit) { return __builtin_popcount(set & ((1u << bit) - 1)); };
+    auto layout = [](uint32_t set) {
+        PassOp lay{};
+        lay.kind = P_LAYOUT;
+        int t = 0;
+        for (int bit = 0; bit < kRegTileBits; ++bit)
+            if ((set >> bit) & 1u) lay.q[t++] = bit;
+        return lay;
+    };
+    std::vector<PassOp> out;
+    out.reserve(in.size() + 8);
+    uint32_t S = kStart;
+    for (size_t i = 0; i < in.size(); ++i) {
+        const uint32_t need = need_of(in[i]);
+        if (need & ~S) {
+            uint32_t ns = need;
+            for (size_t k = i + 1; k < in.size() && __builtin_popcount(ns) < kRegBits; ++k) {
+                const uint32_t nk = need_of(in[k]);
+                if (__builtin_popcount(ns | nk) <= kRegBits) ns |= nk;
+            }
+            for (int bit = kRegTileBits - 1; bit >= 0 && __builtin_popcount(ns) < kRegBits; --bit)
+                if ((S >> bit) & 1u) ns |= 1u << bit;
+            for (int bit = kRegTileBits - 1; bit >= 0 && __builtin_popcount(ns) < kRegBits; --bit) ns |= 1u << bit;
+            out.push_back(layout(ns));
+            S = ns;
+        }
+        PassOp po = in[i];
+        if (po.kind == P_1Q) {
+            po.kind = R_1Q;
+            po.a = rank(S, in[i].a);
+        } else if (po.kind == P_CNOT) {
+            po.kind = R_CNOT;
+            po.a = in[i].a;
+            po.b = rank(S, in[i].b);
+        }
+        out.push_back(po);
+    }
+    if (S & low) out.push_back(layout(kStart));
+    return out;
+}
+
+}  // namespace
+
 int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
     const int n = s->n;
     std::vector<Checked> info(n_ops);
@@ -315,25 +547,8 @@ int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
 
     const int b = std::min(n, device_tile_bits(s));
     const int L = std::min(kLowBits, b);
+    const bool reg_tiles = (b == kRegTileBits);
     const std::vector<PlannedPass> passes = plan(n, ops, info, b, L);
-
-    // Pack every tiled pass's ops and the dense matrices into one upload.
-    size_t op_count = 0, mat_v2d = 0;
-    for (const PlannedPass& p : passes) {
-        const bool streamed = p.ops.size() == 1 && info[p.ops[0]].single && n > b;
-        if (streamed) continue;
-        op_count += p.ops.size();
-        for (uint64_t i : p.ops)
-            if (ops[i].kind == LLSV_OP_DENSE) mat_v2d += size_t(1) << (2 * ops[i].nq);
-    }
-    const size_t ops_bytes = op_count * sizeof(PassOp);
-    const size_t mat_off_bytes = (ops_bytes + 255) & ~size_t(255);
-    const size_t total = mat_off_bytes + mat_v2d * sizeof(v2d);
-    if (total) {
-        if (int rc = ensure_capacity(s, total)) return rc;
-    }
-    PassOp* hops = reinterpret_cast<PassOp*>(s->program->host);
-    v2d* hmats = total ? reinterpret_cast<v2d*>(s->program->host + mat_off_bytes) : nullptr;
 
     struct Launch {
         bool streamed;
@@ -344,7 +559,9 @@ int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
     };
     std::vector<Launch> launches;
     launches.reserve(passes.size());
-    size_t oi = 0, mi = 0;
+    std::vector<PassOp> all_ops;
+    std::vector<const llsv_op*> dense_src;  // dense ops in upload order
+    size_t mat_v2d = 0;
     for (const PlannedPass& p : passes) {
         Launch la{};
         if (p.ops.size() == 1 && info[p.ops[0]].single && n > b) {
@@ -357,7 +574,6 @@ int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
         d.n = n;
         d.b = b;
         d.L = L;
-        d.nops = int(p.ops.size());
         int local_of[64];
         for (int q = 0; q < 64; ++q) local_of[q] = -1;
         int j = 0;
@@ -369,7 +585,8 @@ int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
             d.pos[t] = d.tq[t];
         }
         std::sort(d.pos, d.pos + b);
-        la.first = oi;
+        std::vector<PassOp> pops;
+        pops.reserve(p.ops.size());
         for (uint64_t i : p.ops) {
             const llsv_op& op = ops[i];
             PassOp po{};
@@ -410,25 +627,40 @@ int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
                     po.kind = P_DENSE;
                     po.a = op.nq;
                     for (int t = 0; t < op.nq; ++t) po.q[t] = local_of[op.q[t]];
-                    po.moff = int(mi);
+                    po.moff = int(mat_v2d);
                     for (int c = 0; c < (1 << op.nq); ++c) {
                         uint32_t o = 0;
                         for (int t = 0; t < op.nq; ++t)
                             if ((c >> t) & 1) o |= 1u << po.q[t];
                         po.dep[c] = o;
                     }
-                    const size_t cnt = size_t(1) << (2 * op.nq);
-                    std::memcpy(hmats + mi, op.dense, cnt * sizeof(v2d));
-                    mi += cnt;
+                    dense_src.push_back(&op);
+                    mat_v2d += size_t(1) << (2 * op.nq);
                     break;
                 }
             }
-            hops[oi++] = po;
+            pops.push_back(po);
         }
+        if (reg_tiles) pops = to_register_program(pops, L);
+        d.nops = int(pops.size());
+        la.first = all_ops.size();
+        all_ops.insert(all_ops.end(), pops.begin(), pops.end());
         launches.push_back(la);
     }
 
-    if (total) {
+    const size_t ops_bytes = all_ops.size() * sizeof(PassOp);
+    const size_t mat_off_bytes = (ops_bytes + 255) & ~size_t(255);
+    const size_t total = mat_off_bytes + mat_v2d * sizeof(v2d);
+    if (all_ops.size()) {
+        if (int rc = ensure_capacity(s, total)) return rc;
+        std::memcpy(s->program->host, all_ops.data(), ops_bytes);
+        v2d* hmats = reinterpret_cast<v2d*>(s->program->host + mat_off_bytes);
+        size_t mi = 0;
+        for (const llsv_op* op : dense_src) {
+            const size_t cnt = size_t(1) << (2 * op->nq);
+            std::memcpy(hmats + mi, op->dense, cnt * sizeof(v2d));
+            mi += cnt;
+        }
         hipError_t e = hipMemcpyAsync(s->program->dev, s->program->host, total, hipMemcpyHostToDevice, s->stream);
         if (e == hipSuccess) e = hipEventRecord(s->program->uploaded, s->stream);
         if (e != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_apply_program: upload: %s", hipGetErrorString(e));
@@ -458,12 +690,19 @@ int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
             }
         } else {
             const uint64_t tiles = uint64_t(1) << (n - b);
-            if (la.dense)
-                hipLaunchKernelGGL((k_pass<kPassThreads, true>), dim3(unsigned(tiles)), dim3(kPassThreads), lds,
-                                   s->stream, s->amps, la.d, dops + la.first, dmats);
-            else
-                hipLaunchKernelGGL((k_pass<kPassThreads, false>), dim3(unsigned(tiles)), dim3(kPassThreads), lds,
-                                   s->stream, s->amps, la.d, dops + la.first, dmats);
+            const dim3 grid{unsigned(tiles), 1, 1}, block{unsigned(kPassThreads), 1, 1};
+            if (reg_tiles) {
+                if (la.dense)
+                    hipLaunchKernelGGL((k_pass_reg<true>), grid, block, lds, s->stream, s->amps, la.d, dops + la.first, dmats);
+                else
+                    hipLaunchKernelGGL((k_pass_reg<false>), grid, block, lds, s->stream, s->amps, la.d, dops + la.first, dmats);
+            } else if (la.dense) {
+                hipLaunchKernelGGL((k_pass<kPassThreads, true>), grid, block, lds, s->stream, s->amps, la.d,
+                                   dops + la.first, dmats);
+            } else {
+                hipLaunchKernelGGL((k_pass<kPassThreads, false>), grid, block, lds, s->stream, s->amps, la.d,
+                                   dops + la.first, dmats);
+            }
             e = hipGetLastError();
         }
         if (e != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_apply_program: launch: %s", hipGetErrorString(e));
