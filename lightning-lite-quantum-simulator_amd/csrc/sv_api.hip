@@ -16,6 +16,7 @@
 
 #include "../../include/llsv.h"
 #include "sv_common.h"
+#include "sv_dense.h"
 #include "sv_kernels.h"
 #include "sv_program.h"
 
@@ -232,8 +233,13 @@ int llsv_apply_dense(llsv_state* s, int k, const int* targets, const double* m) 
     const size_t ubytes = size_t(D) * D * sizeof(v2d);
     LLSV_HIP(hipMemcpyAsync(s->params, m, ubytes, hipMemcpyHostToDevice, s->stream));
     LLSV_HIP(hipMemcpyAsync(s->params + ubytes, off, D * sizeof(uint64_t), hipMemcpyHostToDevice, s->stream));
-    LLSV_HIP(launch_dense(s->stream, s->amps, s->n, k, b, reinterpret_cast<const v2d*>(s->params),
-                          reinterpret_cast<const uint64_t*>(s->params + ubytes)));
+    PassDesc pd;
+    DenseTile dt;
+    if (make_dense_tile(s->n, k, targets, &pd, &dt))
+        LLSV_HIP(launch_dense_mfma(s->stream, s->amps, pd, dt, reinterpret_cast<const v2d*>(s->params)));
+    else
+        LLSV_HIP(launch_dense(s->stream, s->amps, s->n, k, b, reinterpret_cast<const v2d*>(s->params),
+                              reinterpret_cast<const uint64_t*>(s->params + ubytes)));
     return LLSV_OK;
 }
 

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <vector>
 
+#include "sv_dense.h"
 #include "sv_kernels.h"
 #include "sv_program.h"
 #include "sv_state.h"
@@ -553,9 +554,12 @@ int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
     struct Launch {
         bool streamed;
         bool dense;
+        bool mfma;  // a lone dense block of 4-5 qubits: the FP64-MFMA kernel
         uint64_t op;
         PassDesc d;
+        DenseTile dt;
         size_t first;
+        size_t moff;
     };
     std::vector<Launch> launches;
     launches.reserve(passes.size());
@@ -567,6 +571,15 @@ int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
         if (p.ops.size() == 1 && info[p.ops[0]].single && n > b) {
             la.streamed = true;
             la.op = p.ops[0];
+            launches.push_back(la);
+            continue;
+        }
+        if (p.ops.size() == 1 && ops[p.ops[0]].kind == LLSV_OP_DENSE &&
+            make_dense_tile(n, ops[p.ops[0]].nq, ops[p.ops[0]].q, &la.d, &la.dt)) {
+            la.mfma = true;
+            la.moff = mat_v2d;
+            dense_src.push_back(&ops[p.ops[0]]);
+            mat_v2d += size_t(1) << (2 * ops[p.ops[0]].nq);
             launches.push_back(la);
             continue;
         }
@@ -651,7 +664,7 @@ int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
     const size_t ops_bytes = all_ops.size() * sizeof(PassOp);
     const size_t mat_off_bytes = (ops_bytes + 255) & ~size_t(255);
     const size_t total = mat_off_bytes + mat_v2d * sizeof(v2d);
-    if (all_ops.size()) {
+    if (all_ops.size() || mat_v2d) {
         if (int rc = ensure_capacity(s, total)) return rc;
         std::memcpy(s->program->host, all_ops.data(), ops_bytes);
         v2d* hmats = reinterpret_cast<v2d*>(s->program->host + mat_off_bytes);
@@ -670,7 +683,9 @@ int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
     const size_t lds = size_t(16) << b;
     for (const Launch& la : launches) {
         hipError_t e = hipSuccess;
-        if (la.streamed) {
+        if (la.mfma) {
+            e = launch_dense_mfma(s->stream, s->amps, la.d, la.dt, dmats + la.moff);
+        } else if (la.streamed) {
             const llsv_op& op = ops[la.op];
             if (op.kind == LLSV_OP_1Q) {
                 Gate2 g;

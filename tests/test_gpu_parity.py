@@ -256,3 +256,34 @@ def test_28q_inverse_roundtrip():
     assert abs(sv.norm() - 1) < 1e-12
     sv.apply_program(inv)
     assert abs(sv.expect_z(4)) < 1e-12 and abs(sv.expect_z(27) - 1) < 1e-12 and abs(sv.norm() - 1) < 1e-12
+
+
+# ------------------------------------------------------------- fused dense blocks (FP64 MFMA for k = 4, 5)
+@pytest.mark.parametrize("n", [12, 16, 20])
+@pytest.mark.parametrize("max_k", [4, 5])
+def test_fused_blocks_mfma(n, max_k):
+    import gate_fusion as GF
+    from llsv import circuits as C
+
+    ops = C.layered_rotations(n, n, 3)
+    fused = GF.fuse_dense(ops, max_k)
+    assert any(op[0] == "dense" and len(op[1]) >= 4 for op in fused)
+    say(f"fused n={n} max_k={max_k}: {len(ops)} ops -> {len(fused)} blocks")
+    ref = O.run(n, ops)
+    assert maxdiff(gpu_run(n, fused), ref) <= TOL
+    assert maxdiff(gpu_run(n, fused, program=True), ref) <= TOL
+
+
+@pytest.mark.parametrize("targets", [[0, 1, 2, 3, 4], [4, 3, 2, 1, 0], [19, 2, 11, 0, 7], [15, 16, 17, 18, 19],
+                                     [5, 9, 13, 17], [0, 19, 1, 18]])
+def test_dense_mfma_target_orders(targets):
+    # operator bit j <-> targets[j], low and high targets, sorted and unsorted
+    n = 20
+    rng = np.random.default_rng(sum(targets))
+    start = (rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n))
+    start /= np.linalg.norm(start)
+    U = random_unitary(rng, 1 << len(targets))
+    sv = qsv().StateVector(start.astype(np.complex128))
+    sv.defer_host_sync(True)
+    sv.apply_fused(targets, U)
+    assert maxdiff(sv.state(), O.run(n, [("dense", targets, U)], psi=start)) <= TOL
