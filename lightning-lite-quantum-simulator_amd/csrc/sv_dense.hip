@@ -13,7 +13,7 @@
 // (no other wavefront reads them).  Then the tile goes back to HBM.
 //
 // MFMA lane layout (checked on gfx950): A[i][k] is lane i + 16 k, B[k][j] is lane j + 16 k,
-// D[i][j] is lane j + 16 (i / 4), element i % 4.
+// D[i][j] is lane j + 16 (i % 4), element i / 4.
 #include "sv_common.h"
 #include "sv_dense.h"
 #include "sv_program.h"
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void k_dense_mfma(v2d* __restrict__ psi, PassD
             an[rt][ks] = -u.y;
         }
     const uint32_t d_in = tdep<K>(dt, lane >> 4);          // column bits 0..1 of a K step
-    const uint32_t d_out = tdep<K>(dt, (lane >> 4) << 2);  // row bits 2..3 of an output block
+    const uint32_t d_out = tdep<K>(dt, lane >> 4);  // row bits 0..1 of an output block
     const uint32_t g_lane = gdep<K>(dt, lane & 15);
     __syncthreads();
 
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(256) void k_dense_mfma(v2d* __restrict__ psi, PassD
                 v2d y;
                 y.x = yr[rt][v];
                 y.y = yi[rt][v];
-                tile[dswz(go | d_out | tdep<K>(dt, uint32_t(v) | (uint32_t(rt) << 4)))] = y;
+                tile[dswz(go | d_out | tdep<K>(dt, (uint32_t(v) << 2) | (uint32_t(rt) << 4)))] = y;
             }
     }
     __syncthreads();
