@@ -25,6 +25,8 @@ import subprocess
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "lightning-lite-quantum-simulator_amd")
 sys.path.insert(0, PKG)
@@ -57,6 +59,89 @@ def load_traffic(n):
         return json.load(f).get("bytes_per_launch")
 
 
+def layered_leg(V3, n=28, layers=20):
+    """28-qubit layered circuit: per layer RX, RY, RZ on every qubit, then a CZ brick.
+    Run unfused (one streamed pass per gate), as one program (LDS passes) and fused into
+    dense <= 5-qubit blocks (FP64 MFMA for 4-5 qubit blocks)."""
+    import gate_fusion as GF
+    from llsv import circuits as C, gates as G
+
+    size = 1 << n
+    ops = C.layered_rotations(1, n, layers)
+    n1 = sum(1 for o in ops if o[0] != "cz")
+    ncz = len(ops) - n1
+    alg_bytes = 32.0 * size * n1 + 8.0 * size * ncz
+    calls = [(o[1], G.matrix(o)) if o[0] != "cz" else (o[1], o[2]) for o in ops]
+    kinds = [o[0] != "cz" for o in ops]
+    sv = V3.StateVector(n)
+
+    def unfused():
+        for one, c in zip(kinds, calls):
+            if one:
+                sv.apply_gate_cache_optimized(c[0], c[1])
+            else:
+                sv.cz(c[0], c[1])
+
+    unfused()
+    sv.sync()
+    t0 = time.perf_counter()
+    unfused()
+    sv.sync()
+    t_unf = time.perf_counter() - t0
+    log(f"layered {n}q unfused: {t_unf:.3f} s")
+
+    from llsv import program as P
+
+    n_passes = len(P.plan(n, ops, sv.program_tile_bits())[1])
+    sv.apply_program(ops)
+    sv.sync()
+    t0 = time.perf_counter()
+    sv.apply_program(ops)
+    sv.sync()
+    t_prog = time.perf_counter() - t0
+    log(f"layered {n}q program: {n_passes} passes, {t_prog:.3f} s")
+
+    th = time.perf_counter()
+    fused = GF.fuse_dense(ops, 5)
+    t_host = time.perf_counter() - th
+    blocks = [(list(o[1]), np.ascontiguousarray(o[2])) if o[0] == "dense" else (o[1], list(o[2])) for o in fused]
+    dense = [o[0] == "dense" for o in fused]
+    flops = sum(8.0 * (1 << len(o[1])) * size for o in fused if o[0] == "dense")
+
+    def run_fused():
+        for d, b in zip(dense, blocks):
+            if d:
+                sv.apply_fused(b[0], b[1])
+            else:
+                sv.apply_gate_cache_optimized(b[0], b[1])
+
+    run_fused()
+    sv.sync()
+    t0 = time.perf_counter()
+    sv.timer_start()
+    run_fused()
+    dev_ms = sv.timer_stop()
+    sv.sync()
+    t_fused = time.perf_counter() - t0
+    k5 = sum(1 for o in fused if o[0] == "dense" and len(o[1]) == 5)
+    log(f"layered {n}q fused: {len(fused)} blocks ({k5} of 5 qubits), {t_fused:.3f} s")
+    del sv
+    return {
+        "workload": f"{n}-qubit complex128 layered circuit, {layers} layers of RX/RY/RZ on every qubit + CZ brick "
+                    f"({n1} rotations, {ncz} CZ)",
+        "unfused": {"s": round(t_unf, 4), "gates_per_s": round(len(ops) / t_unf, 1),
+                    "hbm_GBps": round(alg_bytes / t_unf / 1e9, 1)},
+        "program": {"s": round(t_prog, 4), "passes": n_passes, "gates_per_s": round(len(ops) / t_prog, 1),
+                    "hbm_GBps": round(32.0 * size * n_passes / t_prog / 1e9, 1),
+                    "effective_GBps": round(alg_bytes / t_prog / 1e9, 1)},
+        "fused_mfma": {"s": round(t_fused, 4), "blocks": len(fused), "blocks_5q": k5,
+                       "fusion_host_s": round(t_host, 3), "gates_per_s": round(len(ops) / t_fused, 1),
+                       "hbm_GBps": round(32.0 * size * len(fused) / (dev_ms / 1e3) / 1e9, 1),
+                       "TFLOPs": round(flops / (dev_ms / 1e3) / 1e12, 2),
+                       "mfma_fraction": round(flops / (dev_ms / 1e3) / 78.6e12, 4)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,6 +149,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 28-qubit layered-circuit leg")
     ap.add_argument("--probe", action="store_true",
                     help="launch each kernel a few times and exit (for rocprofv3 counter runs)")
     a = ap.parse_args()
@@ -219,6 +305,8 @@ def main():
         },
     }
     del sv
+    if not a.no_extra:
+        line["layered28"] = layered_leg(V3)
     if not a.no_cpu_baseline:
         log("cpu baseline (subprocess)")
         line["cpu_baseline"] = cpu_baseline(n)
