@@ -147,6 +147,55 @@ int llsv_sync(llsv_state* s);
  * way the reference copies a state, through the host. */
 int llsv_copy_state(llsv_state* dst, llsv_state* src);
 
+/* ---- batched parameterised circuits: energies and adjoint gradients ---------- */
+
+/* One gate of a parameterised circuit.  RX/RY/RZ take their angle from the parameter
+ * vector (params[b * n_params + param], b = batch index); LLSV_PG_1Q is a fixed 2x2 in m. */
+typedef enum {
+    LLSV_PG_RX = 0,
+    LLSV_PG_RY = 1,
+    LLSV_PG_RZ = 2,
+    LLSV_PG_1Q = 3,   /* q0 = target, m = row-major 2x2 */
+    LLSV_PG_CNOT = 4, /* q0 = control, q1 = target */
+    LLSV_PG_CZ = 5    /* q0, q1 */
+} llsv_pgate_kind;
+
+typedef struct {
+    int32_t kind;
+    int32_t q0, q1;
+    int32_t param; /* parameter index for RX/RY/RZ, else ignored */
+    double m[8];
+} llsv_pgate;
+
+/* Term coeff * P of a Hamiltonian H = sum of terms; P = prod over qubits of X (x bit only),
+ * Y (x and z bits) or Z (z bit only). */
+typedef struct {
+    double coeff;
+    uint64_t xmask, zmask;
+} llsv_pauli;
+
+/* Device workspace llsv_pc_adjoint needs for n qubits, `batch` parameter vectors and a circuit
+ * of n_gates gates: two batches of states (psi, lambda) plus reduction partials.
+ * llsv_pc_energy needs less (one batch of states) and accepts this size too. */
+uint64_t llsv_pc_workspace_bytes(int n_qubits, int batch, int n_gates);
+
+/* energies[b] = <psi_b|H|psi_b>, psi_b = U(params_b)|0...0>, for b < batch.  params (batch x
+ * n_params doubles), energies (batch doubles) and workspace are device pointers; work is
+ * queued on `stream` (a hipStream_t, NULL = default stream).  The energy of the reference's
+ * VQE loop, E(theta) (src/Algorithms.h:205-213), evaluated for a whole batch at once. */
+int llsv_pc_energy(int n_qubits, int batch, const llsv_pgate* gates, int n_gates, int n_params,
+                   const double* params, const llsv_pauli* terms, int n_terms, double* energies,
+                   void* workspace, uint64_t workspace_bytes, void* stream);
+
+/* Energies and gradients grads[b * n_params + p] = dE_b / dparam_p by the adjoint method
+ * (one forward sweep, lambda = H psi, one backward sweep applying U_k^dagger to psi and
+ * lambda; dE/dtheta_k = Im <lambda_k| G_k |psi_k> for U_k = exp(-i theta_k G_k / 2)).
+ * Replaces the parameter-shift loop of VQE::optimize (src/Algorithms.h:277-292), which costs
+ * 2 n_params circuit evaluations per gradient. */
+int llsv_pc_adjoint(int n_qubits, int batch, const llsv_pgate* gates, int n_gates, int n_params,
+                    const double* params, const llsv_pauli* terms, int n_terms, double* energies,
+                    double* grads, void* workspace, uint64_t workspace_bytes, void* stream);
+
 /* ---- interop and measurement ------------------------------------------------ */
 
 /* Device address of the amplitudes and the state's stream (hipStream_t as void*). */
@@ -164,6 +213,9 @@ int llsv_timer_stop(llsv_state* s, float* ms);
 #define LLSV_MARK_SLOTS 256
 int llsv_mark(llsv_state* s, int slot);
 int llsv_mark_elapsed(llsv_state* s, int a, int b, float* ms);
+
+/* Wait for all work queued on `stream` (a hipStream_t; NULL = the default stream). */
+int llsv_stream_sync(void* stream);
 
 /* Message of the last failure on this thread ("" if none). */
 const char* llsv_last_error(void);

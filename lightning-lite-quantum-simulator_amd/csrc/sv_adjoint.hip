@@ -1,0 +1,393 @@
+// sv_adjoint.hip -- batched parameterised circuits: energies and adjoint gradients.
+//
+// A batch of B parameter vectors shares one circuit.  Every kernel handles one gate for all
+// B states at once (blockIdx.y = batch index, states B x 2^n contiguous in the workspace),
+// so a circuit of G gates is G launches regardless of B.  Rotation angles are read from the
+// parameter matrix on the device and turned into (cos, sin) in the kernel, so the same
+// launches serve any parameter values (torch tensors, shifted copies for parameter shift).
+//
+// Adjoint sweep (for U_k = exp(-i theta_k G_k / 2), G in {X, Y, Z}):
+//   psi = U_G ... U_1 |0>;  lambda = H psi;  E = Re <psi|lambda>
+//   for k = G..1:  g_k = Im <lambda|G_k|psi>;  psi <- U_k^dag psi;  lambda <- U_k^dag lambda
+// The inner product of step k and both inverse gates are one pass over (psi, lambda).
+// Reductions use a fixed grid and fixed trees: gradients are bitwise reproducible.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "../../include/llsv.h"
+#include "sv_common.h"
+
+namespace llsv {
+
+namespace {
+
+constexpr int kRedGrid = 128;  // blocks per state for kernels that reduce
+
+struct PG {
+    int kind, q0, q1, param;
+    double m[8];
+};
+static_assert(sizeof(PG) == sizeof(llsv_pgate), "llsv_pgate layout");
+
+struct PauliDev {
+    double coeff;
+    uint64_t x, z;
+};
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double bsum(double v, double* lds4) {
+    v = wsum(v);
+    if ((threadIdx.x & 63) == 0) lds4[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) s = (lds4[0] + lds4[1]) + (lds4[2] + lds4[3]);
+    __syncthreads();
+    return s;
+}
+
+// The 2x2 of gate g for batch element b (or its adjoint).
+__device__ __forceinline__ void gate_matrix(const PG& g, const double* __restrict__ params, int P, int b,
+                                            bool dagger, double* m) {
+    if (g.kind == LLSV_PG_1Q) {
+        if (!dagger) {
+            for (int i = 0; i < 8; ++i) m[i] = g.m[i];
+        } else {  // conjugate transpose
+            m[0] = g.m[0], m[1] = -g.m[1];
+            m[2] = g.m[4], m[3] = -g.m[5];
+            m[4] = g.m[2], m[5] = -g.m[3];
+            m[6] = g.m[6], m[7] = -g.m[7];
+        }
+        return;
+    }
+    const double th = params[size_t(b) * P + g.param];
+    double s, c;
+    sincos(0.5 * th, &s, &c);
+    if (dagger) s = -s;  // U(theta)^dagger = U(-theta) for these rotations
+    if (g.kind == LLSV_PG_RX) {
+        m[0] = c, m[1] = 0, m[2] = 0, m[3] = -s, m[4] = 0, m[5] = -s, m[6] = c, m[7] = 0;
+    } else if (g.kind == LLSV_PG_RY) {
+        m[0] = c, m[1] = 0, m[2] = -s, m[3] = 0, m[4] = s, m[5] = 0, m[6] = c, m[7] = 0;
+    } else {  // RZ
+        m[0] = c, m[1] = -s, m[2] = 0, m[3] = 0, m[4] = 0, m[5] = 0, m[6] = c, m[7] = s;
+    }
+}
+
+__device__ __forceinline__ v2d r2(const double* r, v2d a0, v2d a1) {
+    v2d o;
+    o.x = r[0] * a0.x - r[1] * a0.y + r[2] * a1.x - r[3] * a1.y;
+    o.y = r[0] * a0.y + r[1] * a0.x + r[2] * a1.y + r[3] * a1.x;
+    return o;
+}
+
+__global__ __launch_bounds__(256) void k_pc_init(v2d* __restrict__ A, uint64_t size) {
+    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    v2d* s = A + uint64_t(blockIdx.y) * size;
+    if (i < size) {
+        v2d v;
+        v.x = (i == 0) ? 1.0 : 0.0;
+        v.y = 0.0;
+        s[i] = v;
+    }
+}
+
+// One-qubit gate on every state of the batch (A, and Lm when TWO).  With RED, first adds
+// Im <lm|G psi> for the gate's generator G into partials[(b * nslot + slot) * gridDim.x + blk].
+template <bool TWO, bool RED>
+__global__ __launch_bounds__(256) void k_pc_1q(v2d* __restrict__ A, v2d* __restrict__ Lm, uint64_t size, PG g,
+                                               const double* __restrict__ params, int P, int dagger,
+                                               double* __restrict__ partials, int nslot, int slot) {
+    __shared__ double lds4[4];
+    const int b = blockIdx.y;
+    const int q = g.q0;
+    double m[8];
+    gate_matrix(g, params, P, b, dagger != 0, m);
+    v2d* psi = A + uint64_t(b) * size;
+    v2d* lam = TWO ? Lm + uint64_t(b) * size : nullptr;
+    const uint64_t npairs = size >> 1;
+    const uint64_t lomask = (uint64_t(1) << q) - 1, stride = uint64_t(1) << q;
+    double acc = 0.0;
+    for (uint64_t p = uint64_t(blockIdx.x) * 256 + threadIdx.x; p < npairs; p += uint64_t(gridDim.x) * 256) {
+        const uint64_t i0 = ((p & ~lomask) << 1) | (p & lomask), i1 = i0 + stride;
+        const v2d a0 = psi[i0], a1 = psi[i1];
+        psi[i0] = r2(m, a0, a1);
+        psi[i1] = r2(m + 4, a0, a1);
+        if (TWO) {
+            const v2d l0 = lam[i0], l1 = lam[i1];
+            if (RED) {
+                // Im of conj(l0) (G a)_0 + conj(l1) (G a)_1
+                if (g.kind == LLSV_PG_RZ) {         // G = Z: (a0, -a1)
+                    acc += (l0.x * a0.y - l0.y * a0.x) - (l1.x * a1.y - l1.y * a1.x);
+                } else if (g.kind == LLSV_PG_RX) {  // G = X: (a1, a0)
+                    acc += (l0.x * a1.y - l0.y * a1.x) + (l1.x * a0.y - l1.y * a0.x);
+                } else {                            // G = Y: (-i a1, i a0)
+                    // Im(conj(l) * (-i a)) = -Re(conj(l) a); Im(conj(l) * (i a)) = Re(conj(l) a)
+                    acc += -(l0.x * a1.x + l0.y * a1.y) + (l1.x * a0.x + l1.y * a0.y);
+                }
+            }
+            lam[i0] = r2(m, l0, l1);
+            lam[i1] = r2(m + 4, l0, l1);
+        }
+    }
+    if (RED) {
+        const double s = bsum(acc, lds4);
+        if (threadIdx.x == 0) partials[(size_t(b) * nslot + slot) * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// CNOT (kind 4) or CZ (kind 5) on every state of the batch (A, and Lm when TWO).
+template <bool TWO>
+__global__ __launch_bounds__(256) void k_pc_2q(v2d* __restrict__ A, v2d* __restrict__ Lm, uint64_t size, int kind,
+                                               int q0, int q1) {
+    const int b = blockIdx.y;
+    const int lo = q0 < q1 ? q0 : q1, hi = q0 < q1 ? q1 : q0;
+    const uint64_t b0 = uint64_t(1) << q0, b1 = uint64_t(1) << q1;
+    const uint64_t quads = size >> 2;
+    const uint64_t p = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (p >= quads) return;
+    const uint64_t i = insert0(insert0(p, lo), hi) | b0;  // control (or first qubit) set
+    for (int arr = 0; arr < (TWO ? 2 : 1); ++arr) {
+        v2d* s = (arr == 0 ? A : Lm) + uint64_t(b) * size;
+        if (kind == LLSV_PG_CNOT) {
+            const v2d x = s[i];
+            s[i] = s[i | b1];
+            s[i | b1] = x;
+        } else {
+            s[i | b1] = -s[i | b1];
+        }
+    }
+}
+
+// lam = H psi (when STORE) and partials of Re <psi|H psi> per batch element.
+template <bool STORE>
+__global__ __launch_bounds__(256) void k_pc_hpsi(const v2d* __restrict__ A, v2d* __restrict__ Lm, uint64_t size,
+                                                 const PauliDev* __restrict__ terms, int nterms,
+                                                 double* __restrict__ partials) {
+    __shared__ double lds4[4];
+    const int b = blockIdx.y;
+    const v2d* psi = A + uint64_t(b) * size;
+    double acc = 0.0;
+    for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < size; i += uint64_t(gridDim.x) * 256) {
+        double lr = 0.0, li = 0.0;
+        for (int t = 0; t < nterms; ++t) {
+            const PauliDev T = terms[t];
+            const uint64_t src = i ^ T.x;
+            const v2d a = psi[src];
+            // phase = i^{nY} (-1)^{popcount(src & z)}, nY = popcount(x & z)
+            const int ny = __popcll(T.x & T.z) & 3;
+            const double sg = (__popcll(src & T.z) & 1) ? -T.coeff : T.coeff;
+            double pr = a.x, pi = a.y;
+            if (ny == 1) { const double t0 = pr; pr = -pi; pi = t0; }
+            else if (ny == 2) { pr = -pr; pi = -pi; }
+            else if (ny == 3) { const double t0 = pr; pr = pi; pi = -t0; }
+            lr += sg * pr;
+            li += sg * pi;
+        }
+        const v2d p = psi[i];
+        acc += p.x * lr + p.y * li;
+        if (STORE) {
+            v2d l;
+            l.x = lr;
+            l.y = li;
+            Lm[uint64_t(b) * size + i] = l;
+        }
+    }
+    const double s = bsum(acc, lds4);
+    if (threadIdx.x == 0) partials[size_t(b) * gridDim.x + blockIdx.x] = s;
+}
+
+// out[b * nslot + s] = sum over nblk partials (fixed order); grid (nslot, B).
+__global__ __launch_bounds__(256) void k_pc_reduce(const double* __restrict__ partials, int nblk, int nslot,
+                                                   double* __restrict__ out) {
+    __shared__ double lds4[4];
+    const size_t row = size_t(blockIdx.y) * nslot + blockIdx.x;
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < nblk; k += 256) acc += partials[row * nblk + k];
+    const double s = bsum(acc, lds4);
+    if (threadIdx.x == 0) out[row] = s;
+}
+
+// grads[b * P + param(s)] += slot_vals[b * nslot + s], slots in order; one thread per b.
+__global__ void k_pc_scatter(const double* __restrict__ slot_vals, const int* __restrict__ slot_param, int nslot,
+                             int P, int B, double* __restrict__ grads) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    for (int p = 0; p < P; ++p) grads[size_t(b) * P + p] = 0.0;
+    for (int s = 0; s < nslot; ++s) grads[size_t(b) * P + slot_param[s]] += slot_vals[size_t(b) * nslot + s];
+}
+
+struct Layout {
+    uint64_t size;
+    size_t states, partials, slots, terms, slotmap, total;
+    int nblk, nslot;
+};
+
+Layout layout_of(int n, int B, int n_gates, int nslot, int nterms) {
+    Layout L{};
+    L.size = uint64_t(1) << n;
+    const uint64_t npairs = L.size >> 1;
+    L.nblk = int(std::min<uint64_t>(kRedGrid, (npairs + 255) / 256));
+    L.nslot = nslot;
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    L.states = al(L.size * 16 * size_t(B));
+    L.partials = al(sizeof(double) * size_t(B) * (size_t(n_gates) + 1) * kRedGrid);
+    L.slots = al(sizeof(double) * size_t(B) * (size_t(n_gates) + 1));
+    L.terms = al(sizeof(PauliDev) * size_t(nterms > 0 ? nterms : 1));
+    L.slotmap = al(sizeof(int) * size_t(n_gates + 1));
+    L.total = 2 * L.states + L.partials + L.slots + L.terms + L.slotmap;
+    return L;
+}
+
+int validate(int n, int B, const llsv_pgate* gates, int G, int P, const double* params, const llsv_pauli* terms,
+             int T, const char* who) {
+    if (n < 1 || n > 30) return fail(LLSV_ERR_ARG, "%s: n_qubits must be in [1, 30]", who);
+    if (B < 1 || B > 65535) return fail(LLSV_ERR_ARG, "%s: batch must be in [1, 65535]", who);
+    if (G < 0 || (G > 0 && !gates)) return fail(LLSV_ERR_ARG, "%s: bad gate list", who);
+    if (P < 0 || (P > 0 && !params)) return fail(LLSV_ERR_ARG, "%s: bad parameter matrix", who);
+    if (T < 1 || T > 4096 || !terms) return fail(LLSV_ERR_ARG, "%s: need 1..4096 Hamiltonian terms", who);
+    const uint64_t all = (uint64_t(1) << n) - 1;
+    for (int t = 0; t < T; ++t)
+        if ((terms[t].xmask | terms[t].zmask) & ~all) return fail(LLSV_ERR_ARG, "%s: term %d acts outside the register", who, t);
+    for (int k = 0; k < G; ++k) {
+        const llsv_pgate& g = gates[k];
+        if (g.kind < LLSV_PG_RX || g.kind > LLSV_PG_CZ) return fail(LLSV_ERR_ARG, "%s: gate %d: unknown kind", who, k);
+        if (g.q0 < 0 || g.q0 >= n) return fail(LLSV_ERR_ARG, "%s: gate %d: qubit outside the register", who, k);
+        if (g.kind >= LLSV_PG_CNOT && (g.q1 < 0 || g.q1 >= n || g.q1 == g.q0))
+            return fail(LLSV_ERR_ARG, "%s: gate %d: bad second qubit", who, k);
+        if (g.kind <= LLSV_PG_RZ && (g.param < 0 || g.param >= P))
+            return fail(LLSV_ERR_ARG, "%s: gate %d: parameter index outside [0, %d)", who, k, P);
+    }
+    return LLSV_OK;
+}
+
+hipError_t apply_gate(hipStream_t st, v2d* A, v2d* Lm, bool two, const Layout& L, int B, const llsv_pgate& g,
+                      const double* params, int P, bool dagger, double* partials, int slot) {
+    const uint64_t npairs = L.size >> 1;
+    if (g.kind >= LLSV_PG_CNOT) {
+        const uint64_t quads = L.size >> 2;
+        const dim3 grid(unsigned((quads + 255) / 256), unsigned(B));
+        if (two)
+            hipLaunchKernelGGL(k_pc_2q<true>, grid, dim3(256), 0, st, A, Lm, L.size, g.kind, g.q0, g.q1);
+        else
+            hipLaunchKernelGGL(k_pc_2q<false>, grid, dim3(256), 0, st, A, Lm, L.size, g.kind, g.q0, g.q1);
+        return hipGetLastError();
+    }
+    PG pg;
+    std::memcpy(&pg, &g, sizeof pg);
+    const bool red = slot >= 0;
+    const unsigned gx = red ? unsigned(L.nblk) : unsigned(std::min<uint64_t>((npairs + 255) / 256, 4096));
+    const dim3 grid(gx, unsigned(B));
+    if (!two)
+        hipLaunchKernelGGL((k_pc_1q<false, false>), grid, dim3(256), 0, st, A, Lm, L.size, pg, params, P, int(dagger),
+                           partials, L.nslot, 0);
+    else if (!red)
+        hipLaunchKernelGGL((k_pc_1q<true, false>), grid, dim3(256), 0, st, A, Lm, L.size, pg, params, P, int(dagger),
+                           partials, L.nslot, 0);
+    else
+        hipLaunchKernelGGL((k_pc_1q<true, true>), grid, dim3(256), 0, st, A, Lm, L.size, pg, params, P, int(dagger),
+                           partials, L.nslot, slot);
+    return hipGetLastError();
+}
+
+int run(int n, int B, const llsv_pgate* gates, int G, int P, const double* params, const llsv_pauli* terms, int T,
+        double* energies, double* grads, void* ws, uint64_t ws_bytes, void* stream, const char* who) {
+    if (int rc = validate(n, B, gates, G, P, params, terms, T, who)) return rc;
+    if (!energies) return fail(LLSV_ERR_ARG, "%s: energies is null", who);
+    const bool adj = grads != nullptr;
+    std::vector<int> slot_param;
+    std::vector<int> slot_of(G, -1);
+    for (int k = 0; k < G; ++k)
+        if (gates[k].kind <= LLSV_PG_RZ) {
+            slot_of[k] = int(slot_param.size());
+            slot_param.push_back(gates[k].param);
+        }
+    const Layout L = layout_of(n, B, G, int(slot_param.size()), T);
+    const uint64_t need = adj ? L.total : L.total - L.states;
+    if (!ws || ws_bytes < need)
+        return fail(LLSV_ERR_ARG, "%s: workspace of %llu bytes needed, %llu given", who, (unsigned long long)need,
+                    (unsigned long long)ws_bytes);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    unsigned char* base = static_cast<unsigned char*>(ws);
+    v2d* A = reinterpret_cast<v2d*>(base);
+    size_t off = L.states;
+    v2d* Lm = adj ? reinterpret_cast<v2d*>(base + off) : nullptr;
+    if (adj) off += L.states;
+    double* partials = reinterpret_cast<double*>(base + off);
+    off += L.partials;
+    double* slots = reinterpret_cast<double*>(base + off);
+    off += L.slots;
+    PauliDev* dterms = reinterpret_cast<PauliDev*>(base + off);
+    off += L.terms;
+    int* dslotmap = reinterpret_cast<int*>(base + off);
+
+#define CK(expr)                                                                                       \
+    do {                                                                                               \
+        const hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess) return fail(LLSV_ERR_DEVICE, "%s: %s: %s", who, #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+    std::vector<PauliDev> ht(T);
+    for (int t = 0; t < T; ++t) ht[t] = PauliDev{terms[t].coeff, terms[t].xmask, terms[t].zmask};
+    CK(hipMemcpyAsync(dterms, ht.data(), sizeof(PauliDev) * T, hipMemcpyHostToDevice, st));
+    if (adj && !slot_param.empty())
+        CK(hipMemcpyAsync(dslotmap, slot_param.data(), sizeof(int) * slot_param.size(), hipMemcpyHostToDevice, st));
+
+    hipLaunchKernelGGL(k_pc_init, dim3(unsigned((L.size + 255) / 256), unsigned(B)), dim3(256), 0, st, A, L.size);
+    CK(hipGetLastError());
+    for (int k = 0; k < G; ++k) CK(apply_gate(st, A, nullptr, false, L, B, gates[k], params, P, false, partials, -1));
+
+    const dim3 rgrid(unsigned(L.nblk), unsigned(B));
+    if (adj)
+        hipLaunchKernelGGL(k_pc_hpsi<true>, rgrid, dim3(256), 0, st, A, Lm, L.size, dterms, T, partials);
+    else
+        hipLaunchKernelGGL(k_pc_hpsi<false>, rgrid, dim3(256), 0, st, A, Lm, L.size, dterms, T, partials);
+    CK(hipGetLastError());
+    hipLaunchKernelGGL(k_pc_reduce, dim3(1, unsigned(B)), dim3(256), 0, st, partials, L.nblk, 1, energies);
+    CK(hipGetLastError());
+    if (!adj) return LLSV_OK;
+
+    for (int k = G - 1; k >= 0; --k)
+        CK(apply_gate(st, A, Lm, true, L, B, gates[k], params, P, true, partials, slot_of[k]));
+    if (!slot_param.empty()) {
+        hipLaunchKernelGGL(k_pc_reduce, dim3(unsigned(L.nslot), unsigned(B)), dim3(256), 0, st, partials, L.nblk,
+                           L.nslot, slots);
+        CK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_pc_scatter, dim3(unsigned((B + 63) / 64)), dim3(64), 0, st, slots, dslotmap, L.nslot, P, B,
+                       grads);
+    CK(hipGetLastError());
+#undef CK
+    return LLSV_OK;
+}
+
+}  // namespace
+}  // namespace llsv
+
+extern "C" {
+
+uint64_t llsv_pc_workspace_bytes(int n_qubits, int batch, int n_gates) {
+    if (n_qubits < 1 || n_qubits > 30 || batch < 1 || n_gates < 0) return 0;
+    return llsv::layout_of(n_qubits, batch, n_gates, n_gates, 4096).total;
+}
+
+int llsv_pc_energy(int n, int batch, const llsv_pgate* gates, int n_gates, int n_params, const double* params,
+                   const llsv_pauli* terms, int n_terms, double* energies, void* ws, uint64_t ws_bytes, void* stream) {
+    return llsv::run(n, batch, gates, n_gates, n_params, params, terms, n_terms, energies, nullptr, ws, ws_bytes,
+                     stream, "llsv_pc_energy");
+}
+
+int llsv_pc_adjoint(int n, int batch, const llsv_pgate* gates, int n_gates, int n_params, const double* params,
+                    const llsv_pauli* terms, int n_terms, double* energies, double* grads, void* ws,
+                    uint64_t ws_bytes, void* stream) {
+    if (!grads) return llsv::fail(LLSV_ERR_ARG, "llsv_pc_adjoint: grads is null");
+    return llsv::run(n, batch, gates, n_gates, n_params, params, terms, n_terms, energies, grads, ws, ws_bytes,
+                     stream, "llsv_pc_adjoint");
+}
+
+}  // extern "C"

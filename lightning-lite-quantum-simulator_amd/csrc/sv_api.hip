@@ -330,6 +330,11 @@ int llsv_copy_state(llsv_state* dst, llsv_state* src) {
     return LLSV_OK;
 }
 
+int llsv_stream_sync(void* stream) {
+    LLSV_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    return LLSV_OK;
+}
+
 int llsv_mark(llsv_state* s, int slot) {
     LLSV_HANDLE(s);
     if (slot < 0 || slot >= LLSV_MARK_SLOTS) return fail(LLSV_ERR_ARG, "llsv_mark: slot %d outside [0, %d)", slot, LLSV_MARK_SLOTS);

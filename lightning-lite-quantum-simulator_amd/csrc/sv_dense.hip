@@ -94,11 +94,16 @@ __global__ __launch_bounds__(256) void k_dense_mfma(v2d* __restrict__ psi, PassD
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
             const v2d x = tile[dswz(go | d_in | tdep<K>(dt, uint32_t(ks) << 2))];
+            // 2 * RT independent accumulator chains, interleaved so back-to-back MFMAs never wait
+            // on each other's result
 #pragma unroll
             for (int rt = 0; rt < RT; ++rt) {
                 yr[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[rt][ks], x.x, yr[rt], 0, 0, 0);
-                yr[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(an[rt][ks], x.y, yr[rt], 0, 0, 0);
                 yi[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[rt][ks], x.x, yi[rt], 0, 0, 0);
+            }
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+                yr[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(an[rt][ks], x.y, yr[rt], 0, 0, 0);
                 yi[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[rt][ks], x.y, yi[rt], 0, 0, 0);
             }
         }

@@ -484,6 +484,28 @@ static constexpr int kLowBits = 3;
 
 namespace {
 
+// prev := next . prev for two one-qubit ops on the same tile bit (next applied after prev).
+// The product is diagonal when both are.
+void merge_1q(PassOp* prev, const PassOp& next) {
+    const double* a = next.m;
+    double* b = prev->m;
+    double r[8];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            double re = 0.0, im = 0.0;
+            for (int k = 0; k < 2; ++k) {
+                const double xr = a[2 * (2 * i + k)], xi = a[2 * (2 * i + k) + 1];
+                const double yr = b[2 * (2 * k + j)], yi = b[2 * (2 * k + j) + 1];
+                re += xr * yr - xi * yi;
+                im += xr * yi + xi * yr;
+            }
+            r[2 * (2 * i + j)] = re;
+            r[2 * (2 * i + j) + 1] = im;
+        }
+    std::memcpy(b, r, sizeof r);
+    prev->kind = (prev->kind == P_DIAG && next.kind == P_DIAG) ? P_DIAG : P_1Q;
+}
+
 // Turns a pass's ops (tile-local coordinates) into the register-resident program: a layout
 // change is inserted before every one-qubit gate or CNOT whose target bit is not held in
 // registers; the new register set takes that bit, then the target bits of the next ops (so
@@ -600,6 +622,10 @@ int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
         std::sort(d.pos, d.pos + b);
         std::vector<PassOp> pops;
         pops.reserve(p.ops.size());
+        // last_1q[bit]: index in pops of the latest one-qubit op on tile bit `bit` that no later op
+        // touching that bit follows; a new one-qubit op on the bit is multiplied into it.
+        int last_1q[kMaxTileBits];
+        for (int t = 0; t < kMaxTileBits; ++t) last_1q[t] = -1;
         for (uint64_t i : p.ops) {
             const llsv_op& op = ops[i];
             PassOp po{};
@@ -651,6 +677,22 @@ int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
                     mat_v2d += size_t(1) << (2 * op.nq);
                     break;
                 }
+            }
+            if (po.kind == P_1Q || po.kind == P_DIAG) {
+                const int bit = po.a;
+                if (last_1q[bit] >= 0) {
+                    PassOp& prev = pops[last_1q[bit]];
+                    merge_1q(&prev, po);
+                    continue;
+                }
+                last_1q[bit] = int(pops.size());
+            } else if (po.kind == P_CNOT) {
+                last_1q[po.a] = last_1q[po.b] = -1;
+            } else if (po.kind == P_MCZ) {
+                for (int t = 0; t < b; ++t)
+                    if ((po.lmask >> t) & 1) last_1q[t] = -1;
+            } else if (po.kind == P_DENSE) {
+                for (int t = 0; t < po.a; ++t) last_1q[po.q[t]] = -1;
             }
             pops.push_back(po);
         }

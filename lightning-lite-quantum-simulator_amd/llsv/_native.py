@@ -45,6 +45,10 @@ SIGNATURES = {
     "llsv_copy_from_host": (_I, [_P, _P, _U64]),
     "llsv_sync": (_I, [_P]),
     "llsv_copy_state": (_I, [_P, _P]),
+    "llsv_stream_sync": (_I, [_P]),
+    "llsv_pc_workspace_bytes": (_U64, [_I, _I, _I]),
+    "llsv_pc_energy": (_I, [_I, _I, _P, _I, _I, _P, _P, _I, _P, _P, _U64, _P]),
+    "llsv_pc_adjoint": (_I, [_I, _I, _P, _I, _I, _P, _P, _I, _P, _P, _P, _U64, _P]),
     "llsv_mark": (_I, [_P, _I]),
     "llsv_mark_elapsed": (_I, [_P, _I, _I, ctypes.POINTER(ctypes.c_float)]),
     "llsv_device_ptr": (_P, [_P]),
@@ -84,6 +88,21 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+def hip_runtimes_loaded():
+    """Paths of the distinct HIP runtime libraries mapped into this process.  torch ships its
+    own copy; when torch is imported before libllsv both resolve to one runtime (same soname)
+    and torch's streams can be handed to libllsv.  With two runtimes they cannot."""
+    paths = set()
+    try:
+        with open("/proc/self/maps") as f:
+            for line in f:
+                if "libamdhip64" in line:
+                    paths.add(os.path.realpath(line.split()[-1]))
+    except OSError:
+        pass
+    return sorted(paths)
 
 
 class LlsvError(RuntimeError):
