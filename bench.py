@@ -142,6 +142,47 @@ def layered_leg(V3, n=28, layers=20):
     }
 
 
+def adjoint_leg(n=20, layers=10, batch=64):
+    """Hardware-efficient ansatz, 10 layers of RY+RZ on every qubit + CNOT ladder (400 params),
+    H = sum a_i Z_i Z_{i+1} + sum b_i X_i; a batch of 64 parameter vectors.  Adjoint gradients
+    vs GPU parameter shift (2 * 400 shifted energies per vector, batched 256 at a time)."""
+    import torch
+
+    from llsv import adjoint as LA, circuits as C
+
+    gates = LA.hea_gates(n, layers)
+    circ = LA.Circuit(n, gates, C.zz_x_hamiltonian(0, n))
+    rng = np.random.Generator(np.random.PCG64(0))
+    th = torch.tensor(rng.uniform(0, 2 * np.pi, (batch, circ.n_params)), dtype=torch.float64, device="cuda")
+    LA.energies_and_grads(circ, th[:2])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e, g = LA.energies_and_grads(circ, th)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    log(f"adjoint: {batch} gradients of {circ.n_params} params at {n} q in {t_adj:.3f} s")
+    nps = 1
+    t0 = time.perf_counter()
+    ps = LA.param_shift_grads(circ, th[:nps])
+    torch.cuda.synchronize()
+    t_ps = time.perf_counter() - t0
+    err = torch.max(torch.abs(ps - g[:nps])).item()
+    log(f"parameter shift: {nps} gradient in {t_ps:.3f} s, max |adjoint - shift| {err:.2e}")
+    ws = circ.workspace_bytes(batch)
+    return {
+        "workload": f"{n}-qubit HEA, {layers} layers RY+RZ + CNOT ladder ({circ.n_params} params), "
+                    f"H = sum a ZZ + sum b X ({circ.n_terms} terms), batch {batch}, complex128",
+        "adjoint_grads_per_s": round(batch / t_adj, 2),
+        "adjoint_s_per_batch": round(t_adj, 4),
+        "param_shift_grads_per_s": round(nps / t_ps, 4),
+        "speedup_vs_param_shift": round((batch / t_adj) / (nps / t_ps), 1),
+        "max_abs_grad_diff_vs_param_shift": err,
+        "workspace_GiB": round(ws / 2**30, 2),
+        # algorithmic traffic: forward G passes over B states + backward G passes over 2B states
+        "adjoint_hbm_GBps": round((len(gates) * 32.0 + len(gates) * 64.0) * (1 << n) * batch / t_adj / 1e9, 1),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +199,8 @@ def main():
               "path is single-device here; sharding is a design note, not a run)", file=sys.stderr)
         sys.exit(2)
 
+    if not a.no_extra:
+        import torch  # noqa: F401  (first, so torch and libllsv share one HIP runtime)
     import quantum_sim_v3 as V3
     from llsv import circuits as C
 
@@ -307,6 +350,7 @@ def main():
     del sv
     if not a.no_extra:
         line["layered28"] = layered_leg(V3)
+        line["adjoint20"] = adjoint_leg()
     if not a.no_cpu_baseline:
         log("cpu baseline (subprocess)")
         line["cpu_baseline"] = cpu_baseline(n)
