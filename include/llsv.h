@@ -196,6 +196,42 @@ int llsv_pc_adjoint(int n_qubits, int batch, const llsv_pgate* gates, int n_gate
                     const double* params, const llsv_pauli* terms, int n_terms, double* energies,
                     double* grads, void* workspace, uint64_t workspace_bytes, void* stream);
 
+/* ---- batched noisy trajectories ---------------------------------------------- */
+
+/* A gate of a noisy circuit: after it, every qubit it touches goes through amplitude damping
+ * AD(gamma_q) and then phase damping PD(lambda_q) (Kraus sets of src/Noise.h:114-122),
+ * unravelled by sampling: jump iff u >= ||K0 psi||^2 (src/TrajectorySolver.h:425). */
+typedef struct {
+    int32_t kind; /* 0 = one-qubit gate m (row-major 2x2), 1 = CNOT(q0 control, q1 target) */
+    int32_t q0, q1, pad;
+    double m[8];
+} llsv_tgate;
+
+/* One trajectory: a circuit (gates[first_gate .. first_gate + n_gates)) on n_qubits qubits with
+ * noise rates gamma[first_rate + q], lambda[first_rate + q], and its Philox4x32-10 stream:
+ * key = seed, counter = (traj low, traj high, gate index, site), site 0/1 = AD/PD on the gate's
+ * first qubit, 2/3 on its second; u = (x >> 11) 2^-53 with x = out0 | out1 << 32. */
+typedef struct {
+    int32_t n_qubits; /* 1..20 */
+    int32_t first_gate;
+    int32_t n_gates;
+    int32_t first_rate;
+    uint64_t traj;
+} llsv_traj_item;
+
+#define LLSV_TRAJ_ZSTRIDE 32
+
+/* Runs every item in one persistent launch (workgroups take items from a work counter, the
+ * largest first) and returns, in host memory: z[i * LLSV_TRAJ_ZSTRIDE + q] = <Z_q> of item i's
+ * final state; jumps[i] (may be NULL) the number of jumps; hashes[i] (may be NULL) an FNV-1a
+ * hash of (gate, site) of every jump, for checking decisions against a CPU run; state (may be
+ * NULL) the final 2^n amplitudes of item 0.  Synchronous.
+ * Batches TrajectorySolver::estimate_observable (src/TrajectorySolver.h:239-280) at the gate
+ * level, beyond its 10-qubit cap. */
+int llsv_traj_run(const llsv_traj_item* items, int n_items, const llsv_tgate* gates, int n_gates,
+                  const double* gamma, const double* lambda, int n_rates, uint64_t seed, double* z,
+                  uint32_t* jumps, uint64_t* hashes, double* state);
+
 /* ---- interop and measurement ------------------------------------------------ */
 
 /* Device address of the amplitudes and the state's stream (hipStream_t as void*). */

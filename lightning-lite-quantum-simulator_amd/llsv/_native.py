@@ -49,6 +49,7 @@ SIGNATURES = {
     "llsv_pc_workspace_bytes": (_U64, [_I, _I, _I]),
     "llsv_pc_energy": (_I, [_I, _I, _P, _I, _I, _P, _P, _I, _P, _P, _U64, _P]),
     "llsv_pc_adjoint": (_I, [_I, _I, _P, _I, _I, _P, _P, _I, _P, _P, _P, _U64, _P]),
+    "llsv_traj_run": (_I, [_P, _I, _P, _I, _P, _P, _I, _U64, _P, _P, _P, _P]),
     "llsv_mark": (_I, [_P, _I]),
     "llsv_mark_elapsed": (_I, [_P, _I, _I, ctypes.POINTER(ctypes.c_float)]),
     "llsv_device_ptr": (_P, [_P]),

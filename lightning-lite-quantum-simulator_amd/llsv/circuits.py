@@ -97,3 +97,29 @@ def zz_x_hamiltonian(seed: int, n: int):
     terms = [(float(a[i]), 0, (1 << i) | (1 << (i + 1))) for i in range(n - 1)]
     terms += [(float(b[i]), 1 << i, 0) for i in range(n)]
     return terms
+
+
+def noisy_brick(seed: int, n: int = 14, layers: int = 8):
+    """Trajectory workload: per layer H on every qubit, RZ(theta ~ U[0, 2pi)) on every qubit,
+    then CNOT(q, q+1) over even q on even layers and odd q on odd layers.  Noise rates
+    gamma_q ~ U[1e-3, 1e-2] per qubit, lambda_q = gamma_q / 2.  Returns (gates, gamma, lam)."""
+    rng = _rng(seed)
+    gamma = rng.uniform(1e-3, 1e-2, size=n)
+    ops = []
+    for layer in range(layers):
+        ops += [("h", q) for q in range(n)]
+        ops += [("rz", q, float(rng.uniform(0.0, 2.0 * math.pi))) for q in range(n)]
+        ops += [("cnot", q, q + 1) for q in range(layer % 2, n - 1, 2)]
+    return ops, gamma, gamma / 2.0
+
+
+def mixed_batch(seed: int, count: int = 2048, n_min: int = 10, n_max: int = 16, layers: int = 8):
+    """`count` noisy brick circuits with n ~ U{n_min..n_max}, each with its own angles and rates.
+    Returns a list of (n, gates, gamma, lam, traj_id)."""
+    rng = _rng(seed)
+    sizes = rng.integers(n_min, n_max + 1, size=count)
+    out = []
+    for i, n in enumerate(sizes):
+        g, gm, lm = noisy_brick(seed * 100003 + i, int(n), layers)
+        out.append((int(n), g, gm, lm, i))
+    return out

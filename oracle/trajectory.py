@@ -1,0 +1,102 @@
+"""CPU oracle for noisy trajectories and their density-matrix limit -- TEST INFRASTRUCTURE ONLY.
+
+* philox4x32_10 / uniform: the counter-based generator of traj_oracle.c (Salmon et al., SC'11).
+* run: one trajectory through traj_oracle.c (gate, then AD and PD on each touched qubit).
+* density_matrix: the same noisy circuit on a 2^n x 2^n density matrix, every gate and Kraus
+  operator embedded as a full operator with np.kron (a brute-force restatement of the block
+  update of src/Noise.cpp:174-258, as its own test does at src/Noise.cpp:513-535).  The ensemble
+  mean of <Z_q> over trajectories converges to diag(rho) expectations.
+
+Gates: ('u', q, g4) and ('cnot', c, t); rates gamma[q], lam[q].
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import oracle as O
+
+
+class TGate(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("q0", ctypes.c_int32), ("q1", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("m", ctypes.c_double * 8)]
+
+
+def _lib():
+    L = O._lib()
+    if not hasattr(L, "_traj_bound"):
+        P = ctypes.c_void_p
+        L.orc_philox4x32_10.argtypes = [P, P, P]
+        L.orc_philox4x32_10.restype = None
+        L.orc_traj_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32]
+        L.orc_traj_uniform.restype = ctypes.c_double
+        L.orc_traj_run.argtypes = [ctypes.c_int, P, ctypes.c_int, P, P, ctypes.c_uint64, ctypes.c_uint64, P, P, P]
+        L.orc_traj_run.restype = ctypes.c_int
+        L._traj_bound = True
+    return L
+
+
+def philox4x32_10(ctr, key):
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    _lib().orc_philox4x32_10(c, k, o)
+    return [int(v) for v in o]
+
+
+def uniform(seed, traj, gate, site):
+    return float(_lib().orc_traj_uniform(seed, traj, gate, site))
+
+
+def pack(gates):
+    arr = (TGate * max(1, len(gates)))()
+    for i, g in enumerate(gates):
+        if g[0] == "cnot":
+            arr[i].kind, arr[i].q0, arr[i].q1 = 1, int(g[1]), int(g[2])
+        else:
+            arr[i].kind, arr[i].q0 = 0, int(g[1])
+            for j, v in enumerate(O.gate4(g)):
+                arr[i].m[2 * j], arr[i].m[2 * j + 1] = complex(v).real, complex(v).imag
+    return arr
+
+
+def run(n, gates, gamma, lam, seed, traj, packed=None, want_state=False):
+    """(z[n], jumps, hash, state or None) for one trajectory."""
+    arr = packed if packed is not None else pack(gates)
+    g = np.ascontiguousarray(gamma, dtype=np.float64)
+    l = np.ascontiguousarray(lam, dtype=np.float64)
+    z = np.zeros(n)
+    h = ctypes.c_uint64(0)
+    st = np.zeros(1 << n, dtype=np.complex128) if want_state else None
+    jumps = _lib().orc_traj_run(n, arr, len(gates), g.ctypes.data_as(ctypes.c_void_p),
+                                l.ctypes.data_as(ctypes.c_void_p), seed, traj, z.ctypes.data_as(ctypes.c_void_p),
+                                ctypes.byref(h), st.ctypes.data_as(ctypes.c_void_p) if want_state else None)
+    return z, jumps, h.value, st
+
+
+def _embed(n, q, K):
+    M = np.array([[1.0 + 0j]])
+    for k in reversed(range(n)):
+        M = np.kron(M, K if k == q else np.eye(2))
+    return M
+
+
+def density_matrix_z(n, gates, gamma, lam):
+    """<Z_q> of the exact noisy evolution (rho <- U rho U^dag, then sum_k K rho K^dag per channel)."""
+    N = 1 << n
+    rho = np.zeros((N, N), dtype=np.complex128)
+    rho[0, 0] = 1.0
+    for g in gates:
+        U = O.full_operator(n, g)
+        rho = U @ rho @ U.conj().T
+        qs = [g[1], g[2]] if g[0] == "cnot" else [g[1]]
+        for q in qs:
+            gm, lm = gamma[q], lam[q]
+            for ks in ([np.diag([1, np.sqrt(1 - gm)]), np.array([[0, np.sqrt(gm)], [0, 0]])],
+                       [np.diag([1, np.sqrt(1 - lm)]), np.diag([0, np.sqrt(lm)])]):
+                E = [_embed(n, q, K.astype(np.complex128)) for K in ks]
+                rho = sum(e @ rho @ e.conj().T for e in E)
+    p = np.real(np.diag(rho))
+    idx = np.arange(N)
+    return np.array([p[((idx >> q) & 1) == 0].sum() - p[((idx >> q) & 1) == 1].sum() for q in range(n)])

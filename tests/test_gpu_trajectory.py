@@ -1,0 +1,82 @@
+"""Batched trajectories on the GPU vs the CPU trajectory oracle (needs an MI355X).
+
+Per trajectory the jump decisions must be identical (same Philox streams: equal jump counts and
+decision hashes) and the final <Z_q> within 1e-12; the ensemble must match the density matrix
+within 4.5 standard errors."""
+import sys
+
+import numpy as np
+import pytest
+
+from llsv import circuits as C
+from oracle import oracle as O
+from oracle import trajectory as T
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def as_u(gates):
+    return [g if g[0] == "cnot" else ("u", g[1], O.gate4(g)) for g in gates]
+
+
+@pytest.mark.parametrize("n,lo,hi,count", [(8, 0.05, 0.3, 64), (14, 1e-3, 1e-2, 48)])
+def test_trajectories_match_cpu_bit_for_bit(n, lo, hi, count):
+    from llsv import trajectory as LT
+
+    gates, _, _ = C.noisy_brick(n, n, 8)
+    rng = np.random.default_rng(n)
+    gamma = rng.uniform(lo, hi, n)
+    lam = gamma / 2
+    say(f"trajectories n={n} count={count}")
+    z, j, h, st = LT.run_trajectories(n, gates, gamma, lam, seed=99, n_traj=count, want_state=True)
+    packed = T.pack(as_u(gates))
+    assert j.sum() > 0
+    for i in range(count):
+        zc, jc, hc, sc = T.run(n, as_u(gates), gamma, lam, 99, i, packed, want_state=(i == 0))
+        assert j[i] == jc and h[i] == hc, f"trajectory {i}: decisions differ"
+        assert np.max(np.abs(z[i, :n] - zc)) < 1e-12
+        if i == 0:
+            assert np.max(np.abs(st - sc)) < 1e-12
+
+
+def test_ensemble_vs_density_matrix():
+    from llsv import trajectory as LT
+
+    n = 6
+    gates, _, _ = C.noisy_brick(5, n, 3)
+    gamma = np.random.default_rng(1).uniform(0.05, 0.2, n)
+    lam = gamma / 2
+    z, _, _ = LT.run_trajectories(n, gates, gamma, lam, seed=3, n_traj=8192)
+    zdm = T.density_matrix_z(n, as_u(gates), gamma, lam)
+    se = z[:, :n].std(0) / np.sqrt(len(z)) + 1e-12
+    assert np.all(np.abs(z[:, :n].mean(0) - zdm) < 4.5 * se)
+
+
+def test_mixed_batch_one_launch():
+    from llsv import trajectory as LT
+
+    batch = C.mixed_batch(4, count=96, n_min=10, n_max=16)
+    # stronger noise so jumps happen
+    batch = [(n, g, gm * 20, lm * 20, t) for (n, g, gm, lm, t) in batch]
+    say("mixed batch 96")
+    z, j, h = LT.run_batch(batch, seed=17)
+    assert j.sum() > 0
+    for i in range(0, 96, 7):
+        n, g, gm, lm, t = batch[i]
+        zc, jc, hc, _ = T.run(n, as_u(g), gm, lm, 17, t)
+        assert j[i] == jc and h[i] == hc
+        assert np.max(np.abs(z[i, :n] - zc)) < 1e-12
+
+
+def test_bad_arguments():
+    from llsv import trajectory as LT
+
+    with pytest.raises(ValueError):
+        LT.run_trajectories(3, [("h", 3)], [0.1] * 3, [0.1] * 3, seed=1, n_traj=2)
+    with pytest.raises(ValueError):
+        LT.run_trajectories(3, [("h", 0)], [1.5] * 3, [0.1] * 3, seed=1, n_traj=2)
