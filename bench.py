@@ -183,6 +183,47 @@ def adjoint_leg(n=20, layers=10, batch=64):
     }
 
 
+def trajectory_leg(n=14, count=4096, mixed=2048):
+    """4096 noisy trajectories of a 14-qubit brick circuit (8 layers of H, RZ, CNOT brick; AD then
+    PD after every gate on each touched qubit) in one persistent launch; a mixed batch of 2048
+    circuits of 10..16 qubits in one launch; and an ensemble check at 8 qubits against the
+    exact density matrix (CPU oracle, as a checker only)."""
+    from llsv import circuits as C, trajectory as LT
+
+    gates, gamma, lam = C.noisy_brick(0, n, 8)
+    LT.run_trajectories(n, gates, gamma, lam, seed=1, n_traj=64)
+    t0 = time.perf_counter()
+    z, j, _ = LT.run_trajectories(n, gates, gamma, lam, seed=1, n_traj=count)
+    t_traj = time.perf_counter() - t0
+    sites = sum(2 if g[0] == "cnot" else 1 for g in gates) * 2
+    log(f"trajectories: {count} x {n} q in {t_traj:.3f} s ({int(j.sum())} jumps)")
+    batch = C.mixed_batch(1, count=mixed)
+    t0 = time.perf_counter()
+    zb, jb, _ = LT.run_batch(batch, seed=2)
+    t_mixed = time.perf_counter() - t0
+    mixed_gates = sum(len(b[1]) for b in batch)
+    log(f"mixed batch: {mixed} circuits of 10..16 q in {t_mixed:.3f} s")
+    # ensemble check at 8 qubits (density matrix on the CPU, checker only)
+    from oracle import oracle as O, trajectory as T
+
+    g8, gm8, lm8 = C.noisy_brick(3, 8, 8)
+    z8, _, _ = LT.run_trajectories(8, g8, gm8, lm8, seed=4, n_traj=count)
+    zdm = T.density_matrix_z_fast(8, [g if g[0] == "cnot" else ("u", g[1], O.gate4(g)) for g in g8], gm8, lm8)
+    se = z8[:, :8].std(0) / np.sqrt(count) + 1e-15
+    dev = float(np.max(np.abs(z8[:, :8].mean(0) - zdm) / se))
+    return {
+        "workload": f"{count} trajectories, {n}-qubit brick circuit, 8 layers ({len(gates)} gates, {sites} noise "
+                    f"channels per trajectory), gamma ~ U[1e-3, 1e-2], lambda = gamma / 2, Philox4x32-10",
+        "s": round(t_traj, 4),
+        "trajectory_gates_per_s": round(count * len(gates) / t_traj, 1),
+        "trajectories_per_s": round(count / t_traj, 1),
+        "jumps": int(j.sum()),
+        "mixed_batch": {"circuits": mixed, "qubits": "10..16", "s": round(t_mixed, 4),
+                        "gates_per_s": round(mixed_gates / t_mixed, 1)},
+        "ensemble_check_8q_max_sigma": round(dev, 2),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -351,6 +392,7 @@ def main():
     if not a.no_extra:
         line["layered28"] = layered_leg(V3)
         line["adjoint20"] = adjoint_leg()
+        line["trajectories14"] = trajectory_leg()
     if not a.no_cpu_baseline:
         log("cpu baseline (subprocess)")
         line["cpu_baseline"] = cpu_baseline(n)

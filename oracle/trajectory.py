@@ -100,3 +100,44 @@ def density_matrix_z(n, gates, gamma, lam):
     p = np.real(np.diag(rho))
     idx = np.arange(N)
     return np.array([p[((idx >> q) & 1) == 0].sum() - p[((idx >> q) & 1) == 1].sum() for q in range(n)])
+
+
+def _apply_rho(rho, n, q, K, Kd):
+    """rho <- K rho K^dag for a 2x2 K on qubit q, rho as a (2,)*2n tensor (ket axes first,
+    most significant qubit first): the per-block update B -> K B K^dag of src/Noise.cpp:174-258."""
+    ax_ket = n - 1 - q
+    ax_bra = 2 * n - 1 - q
+    rho = np.moveaxis(np.tensordot(K, rho, axes=([1], [ax_ket])), 0, ax_ket)
+    rho = np.moveaxis(np.tensordot(rho, Kd, axes=([ax_bra], [0])), -1, ax_bra)
+    return rho
+
+
+def density_matrix_z_fast(n, gates, gamma, lam):
+    """Same result as density_matrix_z, O(4^n) per operator (n up to ~12)."""
+    rho = np.zeros((2,) * (2 * n), dtype=np.complex128)
+    rho[(0,) * (2 * n)] = 1.0
+    X = np.array([[0, 1], [1, 0]], dtype=np.complex128)
+    for g in gates:
+        if g[0] == "cnot":
+            c, t = g[1], g[2]
+            # CNOT = |0><0|_c (x) I + |1><1|_c (x) X_t, applied on ket and bra
+            for side in (0, 1):
+                ac = (n - 1 - c) + side * n
+                at = (n - 1 - t) + side * n
+                idx = [slice(None)] * (2 * n)
+                idx[ac] = 1
+                sub = rho[tuple(idx)]
+                at2 = at - (1 if at > ac else 0)
+                rho[tuple(idx)] = np.flip(sub, axis=at2)
+            qs = [c, t]
+        else:
+            M = np.asarray(O.gate4(g), dtype=np.complex128).reshape(2, 2)
+            rho = _apply_rho(rho, n, g[1], M, M.conj().T)
+            qs = [g[1]]
+        for q in qs:
+            for ks in ([np.diag([1, np.sqrt(1 - gamma[q])]), np.array([[0, np.sqrt(gamma[q])], [0, 0]])],
+                       [np.diag([1, np.sqrt(1 - lam[q])]), np.diag([0, np.sqrt(lam[q])])]):
+                rho = sum(_apply_rho(rho, n, q, K.astype(np.complex128), K.conj().T.astype(np.complex128)) for K in ks)
+    p = np.real(np.diagonal(rho.reshape(1 << n, 1 << n)))
+    idx = np.arange(1 << n)
+    return np.array([p[((idx >> q) & 1) == 0].sum() - p[((idx >> q) & 1) == 1].sum() for q in range(n)])
