@@ -14,6 +14,8 @@
 //
 // MFMA lane layout (checked on gfx950): A[i][k] is lane i + 16 k, B[k][j] is lane j + 16 k,
 // D[i][j] is lane j + 16 (i % 4), element i / 4.
+#include <algorithm>
+
 #include "sv_common.h"
 #include "sv_dense.h"
 #include "sv_program.h"
@@ -51,7 +53,7 @@ __device__ __forceinline__ uint32_t gdep(const DenseTile& t, uint32_t g) {
 
 template <int K>
 __global__ __launch_bounds__(256) void k_dense_mfma(v2d* __restrict__ psi, PassDesc d, DenseTile dt,
-                                                    const v2d* __restrict__ U) {
+                                                    const v2d* __restrict__ U, uint32_t ntiles) {
     constexpr int D = 1 << K;
     constexpr int RT = D / 16;  // 16-row blocks of U
     constexpr int KS = D / 4;   // 4-column K steps
@@ -59,14 +61,17 @@ __global__ __launch_bounds__(256) void k_dense_mfma(v2d* __restrict__ psi, PassD
     extern __shared__ v2d tile[];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
-    uint64_t base = blockIdx.x;
-    for (int j = 0; j < kDenseTileBits; ++j) base = insert0(base, d.pos[j]);
+    // Persistent: this workgroup walks tiles blockIdx.x, + gridDim.x, ...  The next tile is
+    // loaded into registers while the current one is on the matrix cores.
+    uint64_t goff[16];  // global offset of element tid + r * 256 within a tile
     const uint64_t off_t = dense_tile_offset(d, tid);
-#pragma unroll 4
-    for (uint32_t r = 0; r < 16; ++r) {
-        const uint32_t l = tid + r * 256;
-        tile[dswz(l)] = ld_nt(psi + (base | off_t | dense_tile_offset(d, r * 256)));
-    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) goff[r] = off_t | dense_tile_offset(d, uint32_t(r) * 256);
+    auto tile_base = [&](uint32_t t) {
+        uint64_t base = t;
+        for (int j = 0; j < kDenseTileBits; ++j) base = insert0(base, d.pos[j]);
+        return base;
+    };
 
     double ar[RT][KS], ai[RT][KS], an[RT][KS];
 #pragma unroll
@@ -78,50 +83,66 @@ __global__ __launch_bounds__(256) void k_dense_mfma(v2d* __restrict__ psi, PassD
             ai[rt][ks] = u.y;
             an[rt][ks] = -u.y;
         }
-    const uint32_t d_in = tdep<K>(dt, lane >> 4);          // column bits 0..1 of a K step
+    const uint32_t d_in = tdep<K>(dt, lane >> 4);   // column bits 0..1 of a K step
     const uint32_t d_out = tdep<K>(dt, lane >> 4);  // row bits 0..1 of an output block
     const uint32_t g_lane = gdep<K>(dt, lane & 15);
-    __syncthreads();
 
-    for (int cb = wave; cb < NCB; cb += 4) {
-        const uint32_t go = g_lane | gdep<K>(dt, uint32_t(cb) << 4);
-        d4 yr[RT], yi[RT];
+    v2d pf[16];
+    uint32_t t = blockIdx.x;
+    if (t < ntiles) {
+        const uint64_t base = tile_base(t);
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt) {
-            yr[rt] = d4{0.0, 0.0, 0.0, 0.0};
-            yi[rt] = d4{0.0, 0.0, 0.0, 0.0};
-        }
-#pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-            const v2d x = tile[dswz(go | d_in | tdep<K>(dt, uint32_t(ks) << 2))];
-            // 2 * RT independent accumulator chains, interleaved so back-to-back MFMAs never wait
-            // on each other's result
-#pragma unroll
-            for (int rt = 0; rt < RT; ++rt) {
-                yr[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[rt][ks], x.x, yr[rt], 0, 0, 0);
-                yi[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[rt][ks], x.x, yi[rt], 0, 0, 0);
-            }
-#pragma unroll
-            for (int rt = 0; rt < RT; ++rt) {
-                yr[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(an[rt][ks], x.y, yr[rt], 0, 0, 0);
-                yi[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[rt][ks], x.y, yi[rt], 0, 0, 0);
-            }
-        }
-#pragma unroll
-        for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                v2d y;
-                y.x = yr[rt][v];
-                y.y = yi[rt][v];
-                tile[dswz(go | d_out | tdep<K>(dt, (uint32_t(v) << 2) | (uint32_t(rt) << 4)))] = y;
-            }
+        for (int r = 0; r < 16; ++r) pf[r] = ld_nt(psi + (base | goff[r]));
     }
-    __syncthreads();
-#pragma unroll 4
-    for (uint32_t r = 0; r < 16; ++r) {
-        const uint32_t l = tid + r * 256;
-        st_nt(psi + (base | off_t | dense_tile_offset(d, r * 256)), tile[dswz(l)]);
+    for (; t < ntiles; t += gridDim.x) {
+        const uint64_t base = tile_base(t);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) tile[dswz(tid + r * 256)] = pf[r];
+        __syncthreads();
+        const uint32_t tn = t + gridDim.x;
+        if (tn < ntiles) {
+            const uint64_t nb = tile_base(tn);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) pf[r] = ld_nt(psi + (nb | goff[r]));
+        }
+        for (int cb = wave; cb < NCB; cb += 4) {
+            const uint32_t go = g_lane | gdep<K>(dt, uint32_t(cb) << 4);
+            d4 yr[RT], yi[RT];
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+                yr[rt] = d4{0.0, 0.0, 0.0, 0.0};
+                yi[rt] = d4{0.0, 0.0, 0.0, 0.0};
+            }
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                const v2d x = tile[dswz(go | d_in | tdep<K>(dt, uint32_t(ks) << 2))];
+                // 2 * RT independent accumulator chains, interleaved so back-to-back MFMAs never
+                // wait on each other's result
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) {
+                    yr[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[rt][ks], x.x, yr[rt], 0, 0, 0);
+                    yi[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[rt][ks], x.x, yi[rt], 0, 0, 0);
+                }
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) {
+                    yr[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(an[rt][ks], x.y, yr[rt], 0, 0, 0);
+                    yi[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[rt][ks], x.y, yi[rt], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    v2d y;
+                    y.x = yr[rt][v];
+                    y.y = yi[rt][v];
+                    tile[dswz(go | d_out | tdep<K>(dt, (uint32_t(v) << 2) | (uint32_t(rt) << 4)))] = y;
+                }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 16; ++r) st_nt(psi + (base | goff[r]), tile[dswz(tid + r * 256)]);
+        __syncthreads();
     }
 }
 
@@ -161,10 +182,17 @@ bool make_dense_tile(int n, int k, const int* targets, PassDesc* d, DenseTile* d
 hipError_t launch_dense_mfma(hipStream_t st, v2d* psi, const PassDesc& d, const DenseTile& dt, const v2d* U_dev) {
     const uint64_t tiles = uint64_t(1) << (d.n - kDenseTileBits);
     const size_t lds = size_t(16) << kDenseTileBits;
+    static int persistent = 0;  // 2 resident workgroups per CU (64 KiB of LDS each)
+    if (!persistent) {
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        persistent = 2 * cus;
+    }
+    const unsigned grid = unsigned(std::min<uint64_t>(tiles, uint64_t(persistent)));
     if (dt.k == 5)
-        hipLaunchKernelGGL(k_dense_mfma<5>, dim3(unsigned(tiles)), dim3(256), lds, st, psi, d, dt, U_dev);
+        hipLaunchKernelGGL(k_dense_mfma<5>, dim3(grid), dim3(256), lds, st, psi, d, dt, U_dev, uint32_t(tiles));
     else
-        hipLaunchKernelGGL(k_dense_mfma<4>, dim3(unsigned(tiles)), dim3(256), lds, st, psi, d, dt, U_dev);
+        hipLaunchKernelGGL(k_dense_mfma<4>, dim3(grid), dim3(256), lds, st, psi, d, dt, U_dev, uint32_t(tiles));
     return hipGetLastError();
 }
 
