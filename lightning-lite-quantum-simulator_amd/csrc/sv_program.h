@@ -56,7 +56,6 @@ struct ProgramCache {
     size_t cap = 0;
     hipEvent_t uploaded = nullptr;  // staging may be rewritten once this has completed
     int max_tile_bits = 0;          // from the device's LDS size, filled on first use
-    int persistent = 0;             // resident workgroups of the register-tile kernel
     ~ProgramCache();
 };
 

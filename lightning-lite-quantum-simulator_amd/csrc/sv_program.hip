@@ -208,43 +208,29 @@ __device__ __forceinline__ void reg_cnot(v2d* a, uint32_t lt, const uint32_t* sb
 
 template <bool DENSE>
 __global__ __launch_bounds__(256) void k_pass_reg(v2d* __restrict__ psi, PassDesc d, const PassOp* __restrict__ ops,
-                                                  const v2d* __restrict__ mats, uint32_t ntiles) {
+                                                  const v2d* __restrict__ mats) {
     extern __shared__ v2d tile[];
     const uint32_t tid = threadIdx.x;
-    // Persistent: tiles blockIdx.x, + gridDim.x, ...; the next tile's amplitudes are loaded into
-    // registers (initial layout) while the current tile's ops run.
-    uint64_t gt0 = 0;
-    for (int j = 0; j < 8; ++j)
-        if ((tid >> j) & 1u) gt0 |= local_to_global(d, j);
-    uint64_t gs0[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) gs0[i] = local_to_global(d, 8 + i);
-    auto tile_base = [&](uint32_t t) {
-        uint64_t base = t;
-        for (int j = 0; j < kRegTileBits; ++j) base = insert0(base, d.pos[j]);
-        return base;
-    };
-    auto load_tile = [&](uint32_t t, v2d* dst) {
-        const uint64_t base = tile_base(t);
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const uint64_t go = ((j & 1) ? gs0[0] : 0) | ((j & 2) ? gs0[1] : 0) | ((j & 4) ? gs0[2] : 0) |
-                                ((j & 8) ? gs0[3] : 0);
-            dst[j] = ld_nt(psi + (base | gt0 | go));
-        }
-    };
-    v2d pf[16];
-    uint32_t t = blockIdx.x;
-    if (t < ntiles) load_tile(t, pf);
-    for (; t < ntiles; t += gridDim.x) {
-    const uint64_t base = tile_base(t);
+    uint64_t base = blockIdx.x;
+    for (int j = 0; j < kRegTileBits; ++j) base = insert0(base, d.pos[j]);
+
     // initial layout: S = local bits 8..11, thread bits 0..7 (coalesced runs of 2^L)
     uint32_t sb[4] = {1u << 8, 1u << 9, 1u << 10, 1u << 11};
     uint32_t lt = tid;
     v2d a[16];
+    {
+        uint64_t gt = 0;
+        for (int j = 0; j < 8; ++j)
+            if ((tid >> j) & 1u) gt |= local_to_global(d, j);
+        uint64_t gs[4];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) a[j] = pf[j];
-    if (t + gridDim.x < ntiles) load_tile(t + gridDim.x, pf);
+        for (int i = 0; i < 4; ++i) gs[i] = local_to_global(d, 8 + i);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const uint64_t go = ((j & 1) ? gs[0] : 0) | ((j & 2) ? gs[1] : 0) | ((j & 4) ? gs[2] : 0) | ((j & 8) ? gs[3] : 0);
+            a[j] = ld_nt(psi + (base | gt | go));
+        }
+    }
 
     double sr = 1.0, si = 0.0;
     for (int i = 0; i < d.nops; ++i) {
@@ -337,7 +323,6 @@ __global__ __launch_bounds__(256) void k_pass_reg(v2d* __restrict__ psi, PassDes
         const uint64_t go = ((j & 1) ? gs[0] : 0) | ((j & 2) ? gs[1] : 0) | ((j & 4) ? gs[2] : 0) | ((j & 8) ? gs[3] : 0);
         st_nt(psi + (base | gt | go), unit ? a[j] : cmul(sr, si, a[j]));
     }
-    }  // tiles
 }
 
 // ---------------------------------------------------------------- host: validation
@@ -437,16 +422,6 @@ std::vector<PlannedPass> plan(int n, const llsv_op* ops, const std::vector<Check
 }
 
 }  // namespace
-
-// Resident register-tile workgroups: 2 per CU (64 KiB of LDS each).
-static int persistent_blocks(llsv_state* s) {
-    if (!s->program->persistent) {
-        int cus = 256;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device) != hipSuccess) cus = 256;
-        s->program->persistent = 2 * cus;
-    }
-    return s->program->persistent;
-}
 
 int device_tile_bits(llsv_state* s) {
     if (s->program->max_tile_bits) return s->program->max_tile_bits;
@@ -774,13 +749,10 @@ int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
             const uint64_t tiles = uint64_t(1) << (n - b);
             const dim3 grid{unsigned(tiles), 1, 1}, block{unsigned(kPassThreads), 1, 1};
             if (reg_tiles) {
-                const dim3 pgrid{unsigned(std::min<uint64_t>(tiles, uint64_t(persistent_blocks(s)))), 1, 1};
                 if (la.dense)
-                    hipLaunchKernelGGL((k_pass_reg<true>), pgrid, block, lds, s->stream, s->amps, la.d, dops + la.first,
-                                       dmats, uint32_t(tiles));
+                    hipLaunchKernelGGL((k_pass_reg<true>), grid, block, lds, s->stream, s->amps, la.d, dops + la.first, dmats);
                 else
-                    hipLaunchKernelGGL((k_pass_reg<false>), pgrid, block, lds, s->stream, s->amps, la.d,
-                                       dops + la.first, dmats, uint32_t(tiles));
+                    hipLaunchKernelGGL((k_pass_reg<false>), grid, block, lds, s->stream, s->amps, la.d, dops + la.first, dmats);
             } else if (la.dense) {
                 hipLaunchKernelGGL((k_pass<kPassThreads, true>), grid, block, lds, s->stream, s->amps, la.d,
                                    dops + la.first, dmats);
