@@ -224,6 +224,61 @@ def trajectory_leg(n=14, count=4096, mixed=2048):
     }
 
 
+def q32_leg(V3, n=32, layers=12):
+    """32 qubits (64 GiB): the Haar + CNOT-ladder sweep streamed gate by gate, and the layered
+    circuit (12 layers) fused into <= 5-qubit dense blocks on FP64 MFMA."""
+    import gate_fusion as GF
+    from llsv import circuits as C, gates as G
+
+    size = 1 << n
+    sv = V3.StateVector(n)
+    ops = C.haar_cnot_sweep(5, n, 1)
+    alg = sum(32.0 * size if o[0] == "u" else 16.0 * size for o in ops)
+
+    def sweep():
+        for o in ops:
+            if o[0] == "u":
+                sv.apply_gate_cache_optimized(o[1], list(o[2]))
+            else:
+                sv.cnot(o[1], o[2])
+
+    sweep()
+    sv.sync()
+    t0 = time.perf_counter()
+    sweep()
+    sv.sync()
+    t_sweep = time.perf_counter() - t0
+    log(f"32q sweep: {len(ops)} gates in {t_sweep:.3f} s")
+    lay = C.layered_rotations(2, n, layers)
+    fused = GF.fuse_dense(lay, 5)
+    flops = sum(8.0 * (1 << len(o[1])) * size for o in fused if o[0] == "dense")
+    blocks = [(o[0] == "dense", (list(o[1]), np.ascontiguousarray(o[2])) if o[0] == "dense" else (o[1], list(o[2])))
+              for o in fused]
+    sv.timer_start()
+    t0 = time.perf_counter()
+    for d, b in blocks:
+        if d:
+            sv.apply_fused(b[0], b[1])
+        else:
+            sv.apply_gate_cache_optimized(b[0], b[1])
+    dev_ms = sv.timer_stop()
+    t_f = time.perf_counter() - t0
+    norm = sv.norm()
+    log(f"32q fused: {len(fused)} blocks in {t_f:.3f} s, norm-1 {norm - 1:.1e}")
+    del sv
+    return {
+        "workload": f"{n}-qubit complex128 (64 GiB): Haar + CNOT-ladder sweep ({len(ops)} gates) and a {layers}-layer "
+                    f"RX/RY/RZ + CZ-brick circuit ({len(lay)} gates) fused into {len(fused)} dense <= 5-qubit blocks",
+        "sweep": {"s": round(t_sweep, 4), "hbm_GBps": round(alg / t_sweep / 1e9, 1),
+                  "pct_of_peak": round(100 * alg / t_sweep / 8e12, 1)},
+        "fused_mfma": {"s": round(t_f, 4), "blocks": len(fused), "TFLOPs": round(flops / (dev_ms / 1e3) / 1e12, 2),
+                       "mfma_fraction": round(flops / (dev_ms / 1e3) / 78.6e12, 4),
+                       "hbm_GBps": round(32.0 * size * len(fused) / (dev_ms / 1e3) / 1e9, 1),
+                       "gates_per_s": round(len(lay) / t_f, 1)},
+        "norm_minus_1": norm - 1,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -231,7 +286,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the 28-qubit layered-circuit leg")
+    ap.add_argument("--no-extra", action="store_true", help="skip the layered, adjoint, trajectory and 32-qubit legs")
+    ap.add_argument("--leg", choices=["layered28", "adjoint20", "trajectories14", "q32"],
+                    help="run only this leg (for profiling) and print its JSON")
     ap.add_argument("--probe", action="store_true",
                     help="launch each kernel a few times and exit (for rocprofv3 counter runs)")
     a = ap.parse_args()
@@ -244,6 +301,14 @@ def main():
         import torch  # noqa: F401  (first, so torch and libllsv share one HIP runtime)
     import quantum_sim_v3 as V3
     from llsv import circuits as C
+
+    if a.leg:
+        import torch  # noqa: F401
+
+        fn = {"layered28": lambda: layered_leg(V3), "adjoint20": adjoint_leg, "trajectories14": trajectory_leg,
+              "q32": lambda: q32_leg(V3)}[a.leg]
+        print(json.dumps({a.leg: fn()}), flush=True)
+        return
 
     n = a.qubits
     size = 1 << n
@@ -393,6 +458,7 @@ def main():
         line["layered28"] = layered_leg(V3)
         line["adjoint20"] = adjoint_leg()
         line["trajectories14"] = trajectory_leg()
+        line["q32"] = q32_leg(V3)
     if not a.no_cpu_baseline:
         log("cpu baseline (subprocess)")
         line["cpu_baseline"] = cpu_baseline(n)

@@ -287,3 +287,28 @@ def test_dense_mfma_target_orders(targets):
     sv.defer_host_sync(True)
     sv.apply_fused(targets, U)
     assert maxdiff(sv.state(), O.run(n, [("dense", targets, U)], psi=start)) <= TOL
+
+
+def test_32q_roundtrip():
+    # 64 GiB state: index math past 2^31 pairs, every kernel family, U then U^dagger
+    n = 32
+    rng = np.random.default_rng(32)
+    ops = [("u", q, random_unitary(rng, 2).reshape(4).tolist()) for q in (0, 3, 17, 30, 31)]
+    ops += [("cnot", 31, 0), ("cnot", 1, 31), ("cz", 0, 31), ("dense", [31, 2, 20, 0, 9], random_unitary(rng, 32))]
+    inv = []
+    for op in reversed(ops):
+        if op[0] == "u":
+            inv.append(("u", op[1], np.asarray(op[2]).reshape(2, 2).conj().T.reshape(4).tolist()))
+        elif op[0] == "dense":
+            inv.append(("dense", op[1], np.asarray(op[2]).conj().T))
+        else:
+            inv.append(op)
+    say("32q roundtrip")
+    sv = qsv().StateVector(n)
+    sv.apply_gate_cache_optimized(6, O.mat_h())
+    gpu_ops = ops + inv
+    for op in gpu_ops[:5]:
+        sv.apply_gate_cache_optimized(op[1], op[2])
+    sv.apply_program(gpu_ops[5:])
+    assert abs(sv.norm() - 1) < 1e-12
+    assert abs(sv.expect_z(6)) < 1e-12 and abs(sv.expect_z(31) - 1) < 1e-12 and abs(sv.expect_z(0) - 1) < 1e-12
