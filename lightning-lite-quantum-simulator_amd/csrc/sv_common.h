@@ -39,6 +39,12 @@ __device__ __host__ __forceinline__ uint64_t insert0(uint64_t x, int pos) {
     return ((x >> pos) << (pos + 1)) | lo;
 }
 
+// Launch grid for `chunks` logical workgroups: kernels loop over chunks with a grid stride, so the
+// grid stays at or below 2^20 workgroups (a dispatch counts work-items in 32 bits).
+inline unsigned grid_for(uint64_t chunks) {
+    return unsigned(chunks < (uint64_t(1) << 20) ? (chunks ? chunks : 1) : (uint64_t(1) << 20));
+}
+
 __device__ __forceinline__ v2d ld_nt(const v2d* p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ void st_nt(v2d* p, v2d v) { __builtin_nontemporal_store(v, p); }
 

@@ -13,8 +13,7 @@ namespace llsv {
 
 // ---------------------------------------------------------------- basis fill
 __global__ __launch_bounds__(256) void k_fill_basis(v2d* __restrict__ psi, uint64_t size, uint64_t index) {
-    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (i < size) {
+    for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < size; i += uint64_t(gridDim.x) * 256) {
         v2d v;
         v.x = (i == index) ? 1.0 : 0.0;
         v.y = 0.0;
@@ -24,16 +23,16 @@ __global__ __launch_bounds__(256) void k_fill_basis(v2d* __restrict__ psi, uint6
 
 hipError_t launch_fill_basis(hipStream_t st, v2d* psi, int n, uint64_t index) {
     const uint64_t size = uint64_t(1) << n;
-    const uint64_t blocks = (size + 255) / 256;
-    hipLaunchKernelGGL(k_fill_basis, dim3(unsigned(blocks)), dim3(256), 0, st, psi, size, index);
+    hipLaunchKernelGGL(k_fill_basis, dim3(grid_for((size + 255) / 256)), dim3(256), 0, st, psi, size, index);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- copy
 // Plain device copy, 4 amplitudes per thread in flight.  Also the calibration kernel for
 // HBM traffic counters: it moves exactly 32 bytes per amplitude.
-__global__ __launch_bounds__(256) void k_copy(v2d* __restrict__ dst, const v2d* __restrict__ src, uint64_t size) {
-    const uint64_t i0 = uint64_t(blockIdx.x) * 1024 + threadIdx.x;
+__global__ __launch_bounds__(256) void k_copy(v2d* __restrict__ dst, const v2d* __restrict__ src, uint64_t size, uint64_t nchunks) {
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint64_t i0 = chunk * 1024 + threadIdx.x;
     v2d v[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u)
@@ -41,11 +40,13 @@ __global__ __launch_bounds__(256) void k_copy(v2d* __restrict__ dst, const v2d* 
 #pragma unroll
     for (int u = 0; u < 4; ++u)
         if (i0 + u * 256 < size) st_nt(dst + i0 + u * 256, v[u]);
+  }
 }
 
 hipError_t launch_copy(hipStream_t st, v2d* dst, const v2d* src, int n) {
     const uint64_t size = uint64_t(1) << n;
-    hipLaunchKernelGGL(k_copy, dim3(unsigned((size + 1023) / 1024)), dim3(256), 0, st, dst, src, size);
+    const uint64_t nch = (size + 1023) / 1024;
+    hipLaunchKernelGGL(k_copy, dim3(grid_for(nch)), dim3(256), 0, st, dst, src, size, nch);
     return hipGetLastError();
 }
 
@@ -57,14 +58,15 @@ hipError_t launch_copy(hipStream_t st, v2d* dst, const v2d* src, int n) {
 // XCD_MAP: workgroups are dealt round-robin to the 8 XCDs; with the remap, XCD k walks the
 // k-th contiguous eighth of the pairs instead of every 8th block.  Measured on MI355X at 30
 // qubits, that order streams ~4% faster for 6 <= q < 20 and ~4% slower for q >= 20.
-__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) { return (b % 8) * (nb / 8) + b / 8; }
+__device__ __forceinline__ uint64_t xcd_block(uint64_t b, uint64_t nb) { return (b % 8) * (nb / 8) + b / 8; }
 
 template <int PPT, bool XCD_MAP>
-__global__ __launch_bounds__(256) void k_apply_1q(v2d* __restrict__ psi, uint64_t npairs, int q, Gate2 g) {
+__global__ __launch_bounds__(256) void k_apply_1q(v2d* __restrict__ psi, uint64_t npairs, int q, Gate2 g, uint64_t nchunks) {
     const uint64_t lomask = (uint64_t(1) << q) - 1;
     const uint64_t stride = uint64_t(1) << q;
-    const unsigned blk = XCD_MAP ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-    const uint64_t p0 = uint64_t(blk) * (256 * PPT) + threadIdx.x;
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint64_t blk = XCD_MAP ? xcd_block(chunk, nchunks) : chunk;
+    const uint64_t p0 = blk * (256 * PPT) + threadIdx.x;
     v2d a0[PPT], a1[PPT];
     uint64_t i0[PPT];
 #pragma unroll
@@ -84,6 +86,7 @@ __global__ __launch_bounds__(256) void k_apply_1q(v2d* __restrict__ psi, uint64_
             st_nt(psi + i0[u] + stride, row2(g.r + 4, a0[u], a1[u]));
         }
     }
+  }
 }
 
 // Target q < 6: the partner of amplitude i (i ^ 2^q) sits in the same wavefront when each
@@ -92,8 +95,9 @@ __global__ __launch_bounds__(256) void k_apply_1q(v2d* __restrict__ psi, uint64_
 // instead would make each store write every other 16 bytes of a line, which costs up to 66%
 // extra HBM write traffic for q <= 2.)
 template <int PPT>
-__global__ __launch_bounds__(256) void k_apply_1q_lane(v2d* __restrict__ psi, uint64_t size, int q, Gate2 g) {
-    const uint64_t i0 = uint64_t(blockIdx.x) * (256 * PPT) + threadIdx.x;
+__global__ __launch_bounds__(256) void k_apply_1q_lane(v2d* __restrict__ psi, uint64_t size, int q, Gate2 g, uint64_t nchunks) {
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint64_t i0 = chunk * (256 * PPT) + threadIdx.x;
     const int m = 1 << q;
     const bool hi = (threadIdx.x >> q) & 1;
     const double* r = hi ? g.r + 4 : g.r;
@@ -108,15 +112,17 @@ __global__ __launch_bounds__(256) void k_apply_1q_lane(v2d* __restrict__ psi, ui
         p.y = __shfl_xor(a[u].y, m, 64);
         if (i0 + u * 256 < size) st_nt(psi + i0 + u * 256, hi ? row2(r, p, a[u]) : row2(r, a[u], p));
     }
+  }
 }
 
 // Diagonal gate diag(1, d1): only the bit-q = 1 half changes (Z, S, T, phase gates).
 template <int PPT>
 __global__ __launch_bounds__(256) void k_phase_upper(v2d* __restrict__ psi, uint64_t npairs, int q, double dr,
-                                                     double di) {
+                                                     double di, uint64_t nchunks) {
     const uint64_t lomask = (uint64_t(1) << q) - 1;
     const uint64_t stride = uint64_t(1) << q;
-    const uint64_t p0 = uint64_t(blockIdx.x) * (256 * PPT) + threadIdx.x;
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint64_t p0 = chunk * (256 * PPT) + threadIdx.x;
     v2d a[PPT];
 #pragma unroll
     for (int u = 0; u < PPT; ++u) {
@@ -128,12 +134,14 @@ __global__ __launch_bounds__(256) void k_phase_upper(v2d* __restrict__ psi, uint
         const uint64_t p = p0 + uint64_t(u) * 256;
         if (p < npairs) st_nt(psi + (((p & ~lomask) << 1) | (p & lomask)) + stride, cmul(dr, di, a[u]));
     }
+  }
 }
 
 // Diagonal gate diag(d0, d1) with d0 != 1: every amplitude is scaled, one linear stream.
 template <int PPT>
-__global__ __launch_bounds__(256) void k_diag_1q(v2d* __restrict__ psi, uint64_t size, int q, Gate2 g) {
-    const uint64_t i0 = uint64_t(blockIdx.x) * (256 * PPT) + threadIdx.x;
+__global__ __launch_bounds__(256) void k_diag_1q(v2d* __restrict__ psi, uint64_t size, int q, Gate2 g, uint64_t nchunks) {
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint64_t i0 = chunk * (256 * PPT) + threadIdx.x;
     v2d a[PPT];
 #pragma unroll
     for (int u = 0; u < PPT; ++u) {
@@ -148,6 +156,7 @@ __global__ __launch_bounds__(256) void k_diag_1q(v2d* __restrict__ psi, uint64_t
             st_nt(psi + i, hi ? cmul(g.r[6], g.r[7], a[u]) : cmul(g.r[0], g.r[1], a[u]));
         }
     }
+  }
 }
 
 constexpr int kPPT = 4;
@@ -160,22 +169,22 @@ hipError_t launch_apply_1q(hipStream_t st, v2d* psi, int n, int q, const Gate2& 
     if (diag && g.r[0] == 1.0 && g.r[1] == 0.0 && q >= kLineBits) {
         if (g.r[6] == 1.0 && g.r[7] == 0.0) return hipSuccess;  // identity
         const uint64_t blocks = (npairs + 256 * kPPT - 1) / (256 * kPPT);
-        hipLaunchKernelGGL(k_phase_upper<kPPT>, dim3(unsigned(blocks)), dim3(256), 0, st, psi, npairs, q, g.r[6],
-                           g.r[7]);
+        hipLaunchKernelGGL(k_phase_upper<kPPT>, dim3(grid_for(blocks)), dim3(256), 0, st, psi, npairs, q, g.r[6],
+                           g.r[7], blocks);
     } else if (diag) {
         const uint64_t size = uint64_t(1) << n;
         const uint64_t blocks = (size + 256 * kPPT - 1) / (256 * kPPT);
-        hipLaunchKernelGGL(k_diag_1q<kPPT>, dim3(unsigned(blocks)), dim3(256), 0, st, psi, size, q, g);
+        hipLaunchKernelGGL(k_diag_1q<kPPT>, dim3(grid_for(blocks)), dim3(256), 0, st, psi, size, q, g, blocks);
     } else if (q < 6 && n >= 6) {
         const uint64_t size = uint64_t(1) << n;
         const uint64_t blocks = (size + 256 * kPPT - 1) / (256 * kPPT);
-        hipLaunchKernelGGL(k_apply_1q_lane<kPPT>, dim3(unsigned(blocks)), dim3(256), 0, st, psi, size, q, g);
+        hipLaunchKernelGGL(k_apply_1q_lane<kPPT>, dim3(grid_for(blocks)), dim3(256), 0, st, psi, size, q, g, blocks);
     } else if (q >= 20 || npairs % (256 * 2 * 8) != 0) {
         const uint64_t blocks = (npairs + 256 * 4 - 1) / (256 * 4);
-        hipLaunchKernelGGL((k_apply_1q<4, false>), dim3(unsigned(blocks)), dim3(256), 0, st, psi, npairs, q, g);
+        hipLaunchKernelGGL((k_apply_1q<4, false>), dim3(grid_for(blocks)), dim3(256), 0, st, psi, npairs, q, g, blocks);
     } else {
         const uint64_t blocks = npairs / (256 * 2);  // a multiple of 8, as the remap needs
-        hipLaunchKernelGGL((k_apply_1q<2, true>), dim3(unsigned(blocks)), dim3(256), 0, st, psi, npairs, q, g);
+        hipLaunchKernelGGL((k_apply_1q<2, true>), dim3(grid_for(blocks)), dim3(256), 0, st, psi, npairs, q, g, blocks);
     }
     return hipGetLastError();
 }
@@ -188,8 +197,9 @@ hipError_t launch_apply_1q(hipStream_t st, v2d* psi, int n, int q, const Gate2& 
 // merge them.
 template <int PPT, bool NT>
 __global__ __launch_bounds__(256) void k_cnot(v2d* __restrict__ psi, uint64_t nswaps, int lo, int hi, uint64_t cb,
-                                              uint64_t tb) {
-    const uint64_t p0 = uint64_t(blockIdx.x) * (256 * PPT) + threadIdx.x;
+                                              uint64_t tb, uint64_t nchunks) {
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint64_t p0 = chunk * (256 * PPT) + threadIdx.x;
     v2d a[PPT], b[PPT];
     uint64_t idx[PPT];
 #pragma unroll
@@ -214,6 +224,7 @@ __global__ __launch_bounds__(256) void k_cnot(v2d* __restrict__ psi, uint64_t ns
             }
         }
     }
+  }
 }
 
 // Control c < 3: the control = 1 amplitudes are interleaved with the others at 16-byte
@@ -221,8 +232,9 @@ __global__ __launch_bounds__(256) void k_cnot(v2d* __restrict__ psi, uint64_t ns
 // target t < 6 pairs through a cross-lane shuffle, t >= 6 through the index pairing of the
 // one-qubit kernel; amplitudes whose control bit is 0 are written back unchanged.
 template <int PPT>
-__global__ __launch_bounds__(256) void k_cnot_lane(v2d* __restrict__ psi, uint64_t size, int c, int t) {
-    const uint64_t i0 = uint64_t(blockIdx.x) * (256 * PPT) + threadIdx.x;
+__global__ __launch_bounds__(256) void k_cnot_lane(v2d* __restrict__ psi, uint64_t size, int c, int t, uint64_t nchunks) {
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint64_t i0 = chunk * (256 * PPT) + threadIdx.x;
     const int m = 1 << t;
     v2d a[PPT];
 #pragma unroll
@@ -236,13 +248,15 @@ __global__ __launch_bounds__(256) void k_cnot_lane(v2d* __restrict__ psi, uint64
         const uint64_t i = i0 + u * 256;
         if (i < size) st_nt(psi + i, ((i >> c) & 1) ? p : a[u]);
     }
+  }
 }
 
 template <int PPT>
-__global__ __launch_bounds__(256) void k_cnot_pairs(v2d* __restrict__ psi, uint64_t npairs, int c, int t) {
+__global__ __launch_bounds__(256) void k_cnot_pairs(v2d* __restrict__ psi, uint64_t npairs, int c, int t, uint64_t nchunks) {
     const uint64_t lomask = (uint64_t(1) << t) - 1;
     const uint64_t stride = uint64_t(1) << t;
-    const uint64_t p0 = uint64_t(blockIdx.x) * (256 * PPT) + threadIdx.x;
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint64_t p0 = chunk * (256 * PPT) + threadIdx.x;
     v2d a0[PPT], a1[PPT];
     uint64_t i0[PPT];
 #pragma unroll
@@ -263,6 +277,7 @@ __global__ __launch_bounds__(256) void k_cnot_pairs(v2d* __restrict__ psi, uint6
             st_nt(psi + i0[u] + stride, sw ? a0[u] : a1[u]);
         }
     }
+  }
 }
 
 hipError_t launch_cnot(hipStream_t st, v2d* psi, int n, int c, int t) {
@@ -271,18 +286,19 @@ hipError_t launch_cnot(hipStream_t st, v2d* psi, int n, int c, int t) {
         const uint64_t nswaps = uint64_t(1) << (n - 2);
         const uint64_t blocks = (nswaps + 256 * kPPT - 1) / (256 * kPPT);
         if (t >= kLineBits)
-            hipLaunchKernelGGL((k_cnot<kPPT, true>), dim3(unsigned(blocks)), dim3(256), 0, st, psi, nswaps, lo, hi,
-                               uint64_t(1) << c, uint64_t(1) << t);
+            hipLaunchKernelGGL((k_cnot<kPPT, true>), dim3(grid_for(blocks)), dim3(256), 0, st, psi, nswaps, lo, hi,
+                               uint64_t(1) << c, uint64_t(1) << t, blocks);
         else
-            hipLaunchKernelGGL((k_cnot<kPPT, false>), dim3(unsigned(blocks)), dim3(256), 0, st, psi, nswaps, lo, hi,
-                               uint64_t(1) << c, uint64_t(1) << t);
+            hipLaunchKernelGGL((k_cnot<kPPT, false>), dim3(grid_for(blocks)), dim3(256), 0, st, psi, nswaps, lo, hi,
+                               uint64_t(1) << c, uint64_t(1) << t, blocks);
     } else if (t < 6) {
         const uint64_t size = uint64_t(1) << n;
-        hipLaunchKernelGGL(k_cnot_lane<kPPT>, dim3(unsigned((size + 1023) / 1024)), dim3(256), 0, st, psi, size, c, t);
+        const uint64_t blocks = (size + 1023) / 1024;
+        hipLaunchKernelGGL(k_cnot_lane<kPPT>, dim3(grid_for(blocks)), dim3(256), 0, st, psi, size, c, t, blocks);
     } else {
         const uint64_t npairs = uint64_t(1) << (n - 1);
-        hipLaunchKernelGGL(k_cnot_pairs<kPPT>, dim3(unsigned((npairs + 1023) / 1024)), dim3(256), 0, st, psi, npairs,
-                           c, t);
+        const uint64_t blocks = (npairs + 1023) / 1024;
+        hipLaunchKernelGGL(k_cnot_pairs<kPPT>, dim3(grid_for(blocks)), dim3(256), 0, st, psi, npairs, c, t, blocks);
     }
     return hipGetLastError();
 }
@@ -291,8 +307,9 @@ hipError_t launch_cnot(hipStream_t st, v2d* psi, int n, int c, int t) {
 // Only the 2^(n-m) amplitudes with all m listed bits set are touched: index p with the
 // m bits removed, bits re-inserted in ascending position order, then OR the mask.
 template <int PPT>
-__global__ __launch_bounds__(256) void k_mcz(v2d* __restrict__ psi, uint64_t count, BitList bits) {
-    const uint64_t p0 = uint64_t(blockIdx.x) * (256 * PPT) + threadIdx.x;
+__global__ __launch_bounds__(256) void k_mcz(v2d* __restrict__ psi, uint64_t count, BitList bits, uint64_t nchunks) {
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint64_t p0 = chunk * (256 * PPT) + threadIdx.x;
     v2d a[PPT];
     uint64_t idx[PPT];
 #pragma unroll
@@ -308,13 +325,15 @@ __global__ __launch_bounds__(256) void k_mcz(v2d* __restrict__ psi, uint64_t cou
         const uint64_t p = p0 + uint64_t(u) * 256;
         if (p < count) st_nt(psi + idx[u], -a[u]);
     }
+  }
 }
 
 // Lowest masked qubit < 3: the touched amplitudes are spread over every line, so stream
 // the whole state once (negating the matching ones) rather than write partial lines.
 template <int PPT>
-__global__ __launch_bounds__(256) void k_mcz_full(v2d* __restrict__ psi, uint64_t size, uint64_t mask) {
-    const uint64_t i0 = uint64_t(blockIdx.x) * (256 * PPT) + threadIdx.x;
+__global__ __launch_bounds__(256) void k_mcz_full(v2d* __restrict__ psi, uint64_t size, uint64_t mask, uint64_t nchunks) {
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint64_t i0 = chunk * (256 * PPT) + threadIdx.x;
     v2d a[PPT];
 #pragma unroll
     for (int u = 0; u < PPT; ++u)
@@ -324,18 +343,19 @@ __global__ __launch_bounds__(256) void k_mcz_full(v2d* __restrict__ psi, uint64_
         const uint64_t i = i0 + u * 256;
         if (i < size) st_nt(psi + i, ((i & mask) == mask) ? -a[u] : a[u]);
     }
+  }
 }
 
 hipError_t launch_mcz(hipStream_t st, v2d* psi, int n, const BitList& bits) {
     if (bits.pos[0] < kLineBits && n >= 6) {
         const uint64_t size = uint64_t(1) << n;
-        hipLaunchKernelGGL(k_mcz_full<kPPT>, dim3(unsigned((size + 1023) / 1024)), dim3(256), 0, st, psi, size,
-                           bits.mask);
+        const uint64_t blocks = (size + 1023) / 1024;
+        hipLaunchKernelGGL(k_mcz_full<kPPT>, dim3(grid_for(blocks)), dim3(256), 0, st, psi, size, bits.mask, blocks);
         return hipGetLastError();
     }
     const uint64_t count = uint64_t(1) << (n - bits.m);
     const uint64_t blocks = (count + 256 * kPPT - 1) / (256 * kPPT);
-    hipLaunchKernelGGL(k_mcz<kPPT>, dim3(unsigned(blocks)), dim3(256), 0, st, psi, count, bits);
+    hipLaunchKernelGGL(k_mcz<kPPT>, dim3(grid_for(blocks)), dim3(256), 0, st, psi, count, bits, blocks);
     return hipGetLastError();
 }
 

@@ -113,8 +113,7 @@ hipError_t launch_inner(hipStream_t st, const v2d* a, const v2d* b, int n, doubl
 
 __global__ __launch_bounds__(256) void k_probabilities(const v2d* __restrict__ psi, uint64_t size,
                                                        double* __restrict__ out) {
-    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
-    if (i < size) {
+    for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < size; i += uint64_t(gridDim.x) * 256) {
         const v2d a = ld_nt(psi + i);
         out[i] = a.x * a.x + a.y * a.y;
     }
@@ -122,7 +121,7 @@ __global__ __launch_bounds__(256) void k_probabilities(const v2d* __restrict__ p
 
 hipError_t launch_probabilities(hipStream_t st, const v2d* psi, int n, double* out_dev) {
     const uint64_t size = uint64_t(1) << n;
-    hipLaunchKernelGGL(k_probabilities, dim3(unsigned((size + 255) / 256)), dim3(256), 0, st, psi, size, out_dev);
+    hipLaunchKernelGGL(k_probabilities, dim3(grid_for((size + 255) / 256)), dim3(256), 0, st, psi, size, out_dev);
     return hipGetLastError();
 }
 
