@@ -224,6 +224,23 @@ def trajectory_leg(n=14, count=4096, mixed=2048):
     }
 
 
+def program_leg(V3, n=28, layers=20):
+    """Only the program path on the 28-qubit layered circuit (for counter runs)."""
+    from llsv import circuits as C, program as P
+
+    ops = C.layered_rotations(1, n, layers)
+    sv = V3.StateVector(n)
+    sv.apply_program(ops)
+    sv.sync()
+    t0 = time.perf_counter()
+    sv.apply_program(ops)
+    sv.sync()
+    t = time.perf_counter() - t0
+    passes = len(P.plan(n, ops, sv.program_tile_bits())[1])
+    del sv
+    return {"passes": passes, "s": round(t, 4), "ms_per_pass": round(t / passes * 1e3, 3)}
+
+
 def q32_leg(V3, n=32, layers=12):
     """32 qubits (64 GiB): the Haar + CNOT-ladder sweep streamed gate by gate, and the layered
     circuit (12 layers) fused into <= 5-qubit dense blocks on FP64 MFMA."""
@@ -287,7 +304,7 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the layered, adjoint, trajectory and 32-qubit legs")
-    ap.add_argument("--leg", choices=["layered28", "adjoint20", "trajectories14", "q32"],
+    ap.add_argument("--leg", choices=["layered28", "program28", "adjoint20", "trajectories14", "q32"],
                     help="run only this leg (for profiling) and print its JSON")
     ap.add_argument("--probe", action="store_true",
                     help="launch each kernel a few times and exit (for rocprofv3 counter runs)")
@@ -305,7 +322,8 @@ def main():
     if a.leg:
         import torch  # noqa: F401
 
-        fn = {"layered28": lambda: layered_leg(V3), "adjoint20": adjoint_leg, "trajectories14": trajectory_leg,
+        fn = {"layered28": lambda: layered_leg(V3), "program28": lambda: program_leg(V3),
+              "adjoint20": adjoint_leg, "trajectories14": trajectory_leg,
               "q32": lambda: q32_leg(V3)}[a.leg]
         print(json.dumps({a.leg: fn()}), flush=True)
         return
