@@ -93,13 +93,24 @@ def layered_leg(V3, n=28, layers=20):
     from llsv import program as P
 
     n_passes = len(P.plan(n, ops, sv.program_tile_bits())[1])
-    sv.apply_program(ops)
-    sv.sync()
-    t0 = time.perf_counter()
-    sv.apply_program(ops)
-    sv.sync()
-    t_prog = time.perf_counter() - t0
-    log(f"layered {n}q program: {n_passes} passes, {t_prog:.3f} s")
+
+    def timed_program():
+        sv.apply_program(ops)
+        sv.sync()
+        t0 = time.perf_counter()
+        sv.apply_program(ops)
+        sv.sync()
+        return time.perf_counter() - t0
+
+    prev = V3.set_program_jit(0)
+    t_interp = timed_program()
+    V3.set_program_jit(prev)
+    tj = time.perf_counter()
+    n_jit = V3.program_jit_precompile(n, ops)
+    t_jit = time.perf_counter() - tj
+    t_prog = timed_program()
+    log(f"layered {n}q program: {n_passes} passes, {t_prog:.3f} s ({n_jit} compiled pass kernels, compile "
+        f"{t_jit:.1f} s; interpreter {t_interp:.3f} s)")
 
     th = time.perf_counter()
     fused = GF.fuse_dense(ops, 5)
@@ -133,7 +144,9 @@ def layered_leg(V3, n=28, layers=20):
                     "hbm_GBps": round(alg_bytes / t_unf / 1e9, 1)},
         "program": {"s": round(t_prog, 4), "passes": n_passes, "gates_per_s": round(len(ops) / t_prog, 1),
                     "hbm_GBps": round(32.0 * size * n_passes / t_prog / 1e9, 1),
-                    "effective_GBps": round(alg_bytes / t_prog / 1e9, 1)},
+                    "effective_GBps": round(alg_bytes / t_prog / 1e9, 1),
+                    "compiled_pass_kernels": n_jit, "compile_s": round(t_jit, 2),
+                    "interpreter_s": round(t_interp, 4)},
         "fused_mfma": {"s": round(t_fused, 4), "blocks": len(fused), "blocks_5q": k5,
                        "fusion_host_s": round(t_host, 3), "gates_per_s": round(len(ops) / t_fused, 1),
                        "hbm_GBps": round(32.0 * size * len(fused) / (dev_ms / 1e3) / 1e9, 1),
@@ -230,6 +243,7 @@ def program_leg(V3, n=28, layers=20):
 
     ops = C.layered_rotations(1, n, layers)
     sv = V3.StateVector(n)
+    n_jit = V3.program_jit_precompile(n, ops)
     sv.apply_program(ops)
     sv.sync()
     t0 = time.perf_counter()
@@ -238,7 +252,7 @@ def program_leg(V3, n=28, layers=20):
     t = time.perf_counter() - t0
     passes = len(P.plan(n, ops, sv.program_tile_bits())[1])
     del sv
-    return {"passes": passes, "s": round(t, 4), "ms_per_pass": round(t / passes * 1e3, 3)}
+    return {"passes": passes, "compiled_pass_kernels": n_jit, "s": round(t, 4), "ms_per_pass": round(t / passes * 1e3, 3)}
 
 
 def q32_leg(V3, n=32, layers=12):
@@ -402,6 +416,9 @@ def main():
 
     tile_bits = sv.program_tile_bits()
     n_passes = len(P.plan(n, ops, tile_bits)[1])
+    tj = time.perf_counter()
+    n_jit = V3.program_jit_precompile(n, ops)
+    t_jit = time.perf_counter() - tj
     sv.apply_program(ops)
     sv.sync()
     t1 = time.perf_counter()
@@ -461,9 +478,11 @@ def main():
         },
         "norm_minus_1": norm - 1,
         "program_path": {
-            "what": "same sweep via apply_program: gates grouped into LDS-staged HBM passes",
+            "what": "same sweep via apply_program: gates grouped into register-tile HBM passes",
             "tile_bits": tile_bits,
             "passes_per_step": n_passes,
+            "compiled_pass_kernels": n_jit,
+            "compile_s": round(t_jit, 2),
             "ms_per_step": round(prog_wall / a.steps * 1e3, 3),
             "hbm_GBps": round(32.0 * size * n_passes * a.steps / prog_wall / 1e9, 1),
             "effective_GBps": round(bytes_step * a.steps / prog_wall / 1e9, 1),

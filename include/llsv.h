@@ -121,6 +121,32 @@ int llsv_plan_passes(int n_qubits, const llsv_op* ops, uint64_t n_ops, int tile_
 /* Tile bits llsv_apply_program uses on this state's device (from the LDS per workgroup). */
 int llsv_program_tile_bits(llsv_state* s);
 
+/* Pass compiler of llsv_apply_program (no counterpart in the reference, which applies gates
+ * one call at a time).  A register-resident pass (12-bit tiles) can run on a kernel generated
+ * for its op sequence and compiled once per process with hipRTC (about a second per distinct
+ * pass structure; coefficients stay run-time data, so changed angles reuse the kernel), or on
+ * the built-in interpreter kernel.  Both give the same amplitudes.  mode 0 = interpreter only,
+ * 1 = passes of states with >= 20 qubits are compiled in a background thread and interpreted
+ * until their kernel is ready (the default; also set by the environment variable
+ * LLSV_PROGRAM_JIT = 0, 1, 2 before first use), 2 = every eligible pass is compiled before its
+ * first launch.
+ * Passes holding dense blocks or more than 256 ops stay on the interpreter.  Returns the
+ * previous mode. */
+int llsv_set_program_jit(int mode);
+
+/* Process-wide counters of the pass compiler: distinct pass kernels compiled, launches of
+ * compiled kernels, compilations that failed (those passes ran on the interpreter). */
+void llsv_program_jit_stats(uint64_t* compiled, uint64_t* launches, uint64_t* failures);
+
+/* Compiles, without a device, the pass kernels llsv_apply_program would use for this program
+ * on an n-qubit state with 12-bit tiles under the current mode, so a later call does not wait
+ * for them.  Returns the number of passes that have a compiled kernel, or an error code. */
+int llsv_program_jit_precompile(int n_qubits, const llsv_op* ops, uint64_t n_ops);
+
+/* Blocks until the pass compiler's background thread (automatic mode) has compiled every
+ * pass requested so far. */
+void llsv_program_jit_wait(void);
+
 /* ---- readout ---------------------------------------------------------------- */
 
 /* sum_i |psi_i|^2 (fixed-order reduction: bitwise reproducible run to run). */

@@ -19,7 +19,7 @@ INC = os.path.join(os.path.dirname(PKG), "include")
 OBJ = os.path.join(PKG, "_build")
 ARCH = os.environ.get("LLSV_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["sv_gates.hip", "sv_reduce.hip", "sv_api.hip", "sv_program.hip", "sv_dense.hip", "sv_adjoint.hip", "sv_traj.hip"]
+HIP_SOURCES = ["sv_gates.hip", "sv_reduce.hip", "sv_api.hip", "sv_program.hip", "sv_dense.hip", "sv_adjoint.hip", "sv_traj.hip", "sv_jit.hip"]
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-Wall", "-Wno-unused-result",
     # packed-FP32 instructions stay off: every kernel here is FP64 anyway.
@@ -54,7 +54,7 @@ def build(verbose: bool = False) -> None:
     _run_parallel(cmds)
     lib = os.path.join(PKG, "libllsv.so")
     tmp = lib + ".tmp"
-    subprocess.run(["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp], check=True)
+    subprocess.run(["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp, "-lhiprtc"], check=True)
     os.replace(tmp, lib)
 
     ext = sysconfig.get_config_var("EXT_SUFFIX")

@@ -130,6 +130,15 @@ public:
         mutate([&] { return llsv_apply_dense(st_, k, targets.data(), mp); });
     }
 
+    // Parses a Python op list (see apply_program); keep holds the dense matrices it points into.
+    static std::vector<llsv_op> parse_ops(const py::list& ops,
+                                          std::vector<py::array_t<cplx, py::array::c_style | py::array::forcecast>>& keep) {
+        std::vector<llsv_op> prog;
+        prog.reserve(py::len(ops));
+        for (const py::handle& h : ops) prog.push_back(parse_op(py::reinterpret_borrow<py::tuple>(h), keep));
+        return prog;
+    }
+
     void apply_program(const py::list& ops) {
         std::vector<llsv_op> prog;
         std::vector<py::array_t<cplx, py::array::c_style | py::array::forcecast>> keep;
@@ -374,5 +383,24 @@ PYBIND11_MODULE(LLSV_PYMOD, m) {
         .def("copy_from", &StateVector::copy_from, py::arg("other"))
         .def("device_ptr", &StateVector::device_ptr)
         .def("stream", &StateVector::stream);
+    // the pass compiler of apply_program (process-wide)
+    m.def("set_program_jit", [](int mode) { const int r = llsv_set_program_jit(mode); check(r < 0 ? r : 0); return r; },
+          py::arg("mode"));
+    m.def("program_jit_stats", [] {
+        uint64_t c = 0, l = 0, f = 0;
+        llsv_program_jit_stats(&c, &l, &f);
+        return py::make_tuple(c, l, f);
+    });
+    m.def("program_jit_wait", [] {
+        py::gil_scoped_release nogil;
+        llsv_program_jit_wait();
+    });
+    m.def("program_jit_precompile", [](int n, const py::list& ops) {
+        std::vector<py::array_t<cplx, py::array::c_style | py::array::forcecast>> keep;
+        const std::vector<llsv_op> prog = StateVector::parse_ops(ops, keep);
+        const int r = llsv_program_jit_precompile(n, prog.data(), prog.size());
+        check(r < 0 ? r : 0);
+        return r;
+    }, py::arg("n"), py::arg("ops"));
     m.attr("__backend__") = llsv_version();
 }

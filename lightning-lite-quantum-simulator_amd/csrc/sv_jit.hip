@@ -1,0 +1,433 @@
+// sv_jit.hip -- generation, compilation (hipRTC) and launch of straight-line pass kernels.
+#include "sv_jit.h"
+
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <deque>
+#include <cstdarg>
+#include <cstddef>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "sv_regops.h"
+
+namespace llsv {
+
+static_assert(sizeof(PassDesc) == 4 * (4 + 2 * kMaxTileBits + 4), "generated kernels restate PassDesc");
+static_assert(offsetof(RegOp, m) % sizeof(double) == 0 && sizeof(RegOp) % sizeof(double) == 0,
+              "generated kernels address RegOp coefficients as doubles");
+
+namespace {
+
+std::atomic<int> g_mode{-1};
+std::atomic<uint64_t> g_compiled{0}, g_launches{0}, g_failures{0};
+
+const char kPrelude[] =
+    "typedef double v2d __attribute__((ext_vector_type(2)));\n"
+    "struct PassDesc { int n, b, L, nops; int tq[12]; int pos[12]; int rq[4]; };\n" LLSV_STRINGIZE(
+        LLSV_REGOPS_SOURCE) "\n";
+
+struct Src {
+    std::string s;
+    void line(const char* fmt, ...) __attribute__((format(printf, 2, 3))) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        s += buf;
+        s += '\n';
+    }
+};
+
+}  // namespace
+
+int jit_mode() {
+    int m = g_mode.load();
+    if (m < 0) {
+        // LLSV_PROGRAM_JIT = 0 (never), 1 (automatic, the default) or 2 (always)
+        const char* e = std::getenv("LLSV_PROGRAM_JIT");
+        m = (e && *e >= '0' && *e <= '2' && !e[1]) ? (*e - '0') : int(kJitAuto);
+        int expect = -1;
+        g_mode.compare_exchange_strong(expect, m);
+        m = g_mode.load();
+    }
+    return m;
+}
+
+int set_jit_mode(int mode) {
+    const int prev = jit_mode();
+    g_mode.store(mode);
+    return prev;
+}
+
+bool jit_wanted(int n, int nops, bool dense) {
+    const int m = jit_mode();
+    if (m == kJitOff || dense || nops > kJitMaxOps) return false;
+    return m == kJitAlways || n >= kJitMinQubits;
+}
+
+std::string jit_source(const RegOp* ops, int nops) {
+    const size_t stride = sizeof(RegOp) / sizeof(double), moff = offsetof(RegOp, m) / sizeof(double);
+    Src o;
+    o.s.reserve(4096 + size_t(nops) * 600);
+    o.line("extern \"C\" __global__ __launch_bounds__(256) void LLSV_KERNEL(v2d* __restrict__ psi, PassDesc d, "
+           "const double* __restrict__ M) {");
+    o.line("__shared__ v2d tile[4096];");
+    o.line("char* const lds = (char*)tile;");
+    o.line("const unsigned int tid = threadIdx.x;");
+    o.line("unsigned long long base = blockIdx.x;");
+    o.line("for (int j = 0; j < 12; ++j) base = ((base >> d.pos[j]) << (d.pos[j] + 1)) | (base & ((1ull << d.pos[j]) - 1ull));");
+    o.line("unsigned long long lg[12];");
+    o.line("#pragma unroll");
+    o.line("for (int j = 0; j < 12; ++j) lg[j] = j < d.L ? (1ull << j) : (1ull << d.tq[j]);");
+    o.line("v2d a[16];");
+    o.line("{ unsigned long long gt = 0;");
+    o.line("#pragma unroll");
+    o.line("for (int j = 0; j < 8; ++j) if ((tid >> j) & 1u) gt |= lg[j];");
+    o.line("#pragma unroll");
+    o.line("for (int j = 0; j < 16; ++j) a[j] = __builtin_nontemporal_load(psi + (base | gt | ((j & 1) ? lg[8] : 0ull) | "
+           "((j & 2) ? lg[9] : 0ull) | ((j & 4) ? lg[10] : 0ull) | ((j & 8) ? lg[11] : 0ull))); }");
+    o.line("unsigned int lt = tid, sl = r_swz(tid) << 4;");
+    o.line("double sr = 1.0, si = 0.0;");
+
+    uint32_t S = 0xF00u;  // register bits of the current layout (the kernel starts with 8..11)
+    bool scaled = false;
+    for (int i = 0; i < nops; ++i) {
+        const RegOp& op = ops[i];
+        const size_t c = size_t(i) * stride + moff;
+        switch (op.kind) {
+            case RO_1Q: {
+                const int K = 1 << op.a;
+                o.line("{ const double* m = M + %zu;", c);
+                for (int j = 0; j < 16; ++j)
+                    if (!(j & K)) o.line("r_pu(a[%d], a[%d], m);", j, j | K);
+                o.line("}");
+                break;
+            }
+            case RO_CNOT: {
+                const int K = 1 << op.a;
+                if (op.tmask == 0) {
+                    for (int j = 0; j < 16; ++j)
+                        if (!(j & K) && ((op.rmask >> j) & 1u))
+                            o.line("{ const v2d t = a[%d]; a[%d] = a[%d]; a[%d] = t; }", j, j, j | K, j | K);
+                } else {
+                    o.line("{ const bool tk = (lt & %uu) != 0u;", op.tmask);
+                    for (int j = 0; j < 16; ++j)
+                        if (!(j & K))
+                            o.line("{ const v2d x = a[%d], y = a[%d]; a[%d] = tk ? y : x; a[%d] = tk ? x : y; }", j, j | K,
+                                   j, j | K);
+                    o.line("}");
+                }
+                break;
+            }
+            case RO_DIAG: {
+                if (op.tmask == 0) {
+                    for (int j = 0; j < 16; ++j) {
+                        if ((op.rmask >> j) & 1u)
+                            o.line("r_sc(a[%d], M[%zu], M[%zu]);", j, c + 6, c + 7);
+                        else if (!op.a)
+                            o.line("r_sc(a[%d], M[%zu], M[%zu]);", j, c + 0, c + 1);
+                    }
+                } else {
+                    o.line("{ const bool hi = (lt & %uu) != 0u;", op.tmask);
+                    o.line("const double dr = hi ? M[%zu] : M[%zu], di = hi ? M[%zu] : M[%zu];", c + 6, c, c + 7, c + 1);
+                    o.line("#pragma unroll");
+                    o.line("for (int j = 0; j < 16; ++j) r_sc(a[j], dr, di); }");
+                }
+                break;
+            }
+            case RO_MCZ: {
+                if (!op.rmask) break;
+                if (op.omask == 0 && op.tmask == 0) {
+                    for (int j = 0; j < 16; ++j)
+                        if ((op.rmask >> j) & 1u) o.line("a[%d] = -a[%d];", j, j);
+                    break;
+                }
+                std::string cond;
+                char buf[128];
+                if (op.omask) {
+                    snprintf(buf, sizeof buf, "(base & %lluull) == %lluull", (unsigned long long)op.omask,
+                             (unsigned long long)op.omask);
+                    cond += buf;
+                }
+                if (op.tmask) {
+                    snprintf(buf, sizeof buf, "%s(lt & %uu) == %uu", cond.empty() ? "" : " && ", op.tmask, op.tmask);
+                    cond += buf;
+                }
+                o.line("{ const unsigned int sg = (%s) ? 0x80000000u : 0u;", cond.c_str());
+                for (int j = 0; j < 16; ++j)
+                    if ((op.rmask >> j) & 1u) o.line("a[%d] = r_ng(a[%d], sg);", j, j);
+                o.line("}");
+                break;
+            }
+            case RO_SCALE:
+                scaled = true;
+                o.line("{ const bool hi = (base >> %d) & 1ull; r_phase(sr, si, hi ? M[%zu] : M[%zu], hi ? M[%zu] : M[%zu]); }",
+                       op.a, c + 6, c, c + 7, c + 1);
+                break;
+            case RO_LAYOUT: {
+                o.line("{");
+                for (int j = 0; j < 16; ++j) o.line("*(v2d*)(lds + (sl ^ %uu)) = a[%d];", op.off0[j], j);
+                o.line("__syncthreads();");
+                o.line("lt = r_ins0(r_ins0(r_ins0(r_ins0(tid, %d), %d), %d), %d);", op.q[0], op.q[1], op.q[2], op.q[3]);
+                o.line("sl = r_swz(lt) << 4;");
+                for (int j = 0; j < 16; ++j) o.line("a[%d] = *(const v2d*)(lds + (sl ^ %uu));", j, op.off1[j]);
+                o.line("__syncthreads();");
+                o.line("}");
+                S = (1u << op.q[0]) | (1u << op.q[1]) | (1u << op.q[2]) | (1u << op.q[3]);
+                break;
+            }
+            default:
+                break;  // dense ops never reach the generator (jit_wanted)
+        }
+    }
+
+    int rq[4], t = 0;
+    for (int bit = 0; bit < kRegTileBits; ++bit)
+        if ((S >> bit) & 1u) rq[t++] = bit;
+    o.line("{ unsigned long long gt = 0;");
+    o.line("#pragma unroll");
+    o.line("for (int j = 0; j < 12; ++j) if ((lt >> j) & 1u) gt |= lg[j];");
+    if (scaled) {
+        o.line("if (sr != 1.0 || si != 0.0) {");
+        o.line("#pragma unroll");
+        o.line("for (int j = 0; j < 16; ++j) r_sc(a[j], sr, si); }");
+    }
+    o.line("#pragma unroll");
+    o.line("for (int j = 0; j < 16; ++j) __builtin_nontemporal_store(a[j], psi + (base | gt | ((j & 1) ? lg[%d] : 0ull) | "
+           "((j & 2) ? lg[%d] : 0ull) | ((j & 4) ? lg[%d] : 0ull) | ((j & 8) ? lg[%d] : 0ull))); }",
+           rq[0], rq[1], rq[2], rq[3]);
+    o.line("}");
+    return o.s;
+}
+
+// A code object holds the kernels of one compile batch; modules are loaded per device on first launch.
+struct CodeObject {
+    std::vector<char> code;
+    std::map<int, hipModule_t> mod;  // kept until the process ends
+};
+
+struct JitEntry {
+    enum : int { kNew, kQueued, kBusy, kReady, kFailed };
+    std::atomic<int> state{kNew};
+    std::string body;  // kernel source, named LLSV_KERNEL
+    std::string name;  // symbol in obj
+    std::shared_ptr<CodeObject> obj;
+    std::string error;
+    std::map<int, hipFunction_t> fn;  // per device
+};
+
+namespace {
+
+std::mutex g_mu;
+std::condition_variable g_cv;
+std::deque<JitEntry*> g_queue;
+int g_busy = 0;
+bool g_stop = false;
+std::thread* g_worker = nullptr;
+uint64_t g_uid = 0;
+
+std::unordered_map<std::string, std::unique_ptr<JitEntry>>& cache() {
+    static auto* c = new std::unordered_map<std::string, std::unique_ptr<JitEntry>>();  // lives to process end
+    return *c;
+}
+
+// Compiles the entries (state kBusy, set by the caller) as one hipRTC program, without the lock.
+void compile_batch(const std::vector<JitEntry*>& batch) {
+    std::string src = kPrelude;
+    std::vector<std::string> names;
+    {
+        std::lock_guard<std::mutex> lock(g_mu);
+        for (size_t i = 0; i < batch.size(); ++i) names.push_back("llsv_pass_" + std::to_string(g_uid++));
+    }
+    for (size_t i = 0; i < batch.size(); ++i) {
+        std::string body = batch[i]->body;
+        body.replace(body.find("LLSV_KERNEL"), 11, names[i]);
+        src += body;
+    }
+    // The process keeps its own cache; keep the code-object manager from writing one to disk.
+    setenv("AMD_COMGR_CACHE", "0", 0);
+    auto obj = std::make_shared<CodeObject>();
+    std::string error;
+    hiprtcProgram prog = nullptr;
+    hiprtcResult r = hiprtcCreateProgram(&prog, src.c_str(), "llsv_pass.hip", 0, nullptr, nullptr);
+    if (r == HIPRTC_SUCCESS) {
+        // CodeGenPrepare and the SLP vectorizer take half the compile time of these long
+        // straight-line kernels and change nothing in the code they produce.
+        const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-fno-slp-vectorize", "-mllvm", "-disable-cgp",
+                              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+        r = hiprtcCompileProgram(prog, int(sizeof opts / sizeof opts[0]), opts);
+        size_t n = 0;
+        if (r == HIPRTC_SUCCESS && hiprtcGetCodeSize(prog, &n) == HIPRTC_SUCCESS && n) {
+            obj->code.resize(n);
+            r = hiprtcGetCode(prog, obj->code.data());
+        } else if (r == HIPRTC_SUCCESS) {
+            r = HIPRTC_ERROR_COMPILATION;
+        }
+        if (r != HIPRTC_SUCCESS) {
+            size_t ls = 0;
+            if (hiprtcGetProgramLogSize(prog, &ls) == HIPRTC_SUCCESS && ls > 1) {
+                std::string log(ls, '\0');
+                if (hiprtcGetProgramLog(prog, &log[0]) == HIPRTC_SUCCESS) error = log.substr(0, 4000);
+            }
+        }
+        (void)hiprtcDestroyProgram(&prog);
+    }
+    if (r != HIPRTC_SUCCESS) error = std::string("hipRTC: ") + hiprtcGetErrorString(r) + (error.empty() ? "" : ": " + error);
+    std::lock_guard<std::mutex> lock(g_mu);
+    for (size_t i = 0; i < batch.size(); ++i) {
+        JitEntry* e = batch[i];
+        if (r == HIPRTC_SUCCESS) {
+            e->obj = obj;
+            e->name = names[i];
+            e->state = JitEntry::kReady;
+            ++g_compiled;
+        } else {
+            e->error = error;
+            e->state = JitEntry::kFailed;
+            ++g_failures;
+        }
+    }
+    g_busy -= int(batch.size());
+    g_cv.notify_all();
+}
+
+void stop_worker() {
+    {
+        std::lock_guard<std::mutex> lock(g_mu);
+        g_stop = true;
+    }
+    g_cv.notify_all();
+    if (g_worker && g_worker->joinable()) g_worker->join();
+}
+
+void worker_loop() {
+    for (;;) {
+        std::vector<JitEntry*> batch;
+        {
+            std::unique_lock<std::mutex> lock(g_mu);
+            g_cv.wait(lock, [] { return g_stop || !g_queue.empty(); });
+            if (g_stop) {
+                for (JitEntry* e : g_queue) e->state = JitEntry::kNew;
+                g_queue.clear();
+                g_cv.notify_all();
+                return;
+            }
+            while (!g_queue.empty() && batch.size() < 64) {
+                JitEntry* e = g_queue.front();
+                g_queue.pop_front();
+                e->state = JitEntry::kBusy;
+                batch.push_back(e);
+            }
+            g_busy += int(batch.size());
+        }
+        compile_batch(batch);
+    }
+}
+
+}  // namespace
+
+JitEntry* jit_entry(const std::string& body) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    auto& c = cache();
+    auto it = c.find(body);
+    if (it == c.end()) {
+        auto e = std::make_unique<JitEntry>();
+        e->body = body;
+        it = c.emplace(body, std::move(e)).first;
+    }
+    return it->second.get();
+}
+
+bool jit_ready(const JitEntry* e) { return e && e->state.load() == JitEntry::kReady; }
+
+void jit_request(const std::vector<JitEntry*>& entries, bool now) {
+    std::vector<JitEntry*> mine;
+    {
+        std::unique_lock<std::mutex> lock(g_mu);
+        for (JitEntry* e : entries) {
+            const int st = e->state.load();
+            if (st == JitEntry::kNew || (now && st == JitEntry::kQueued)) {
+                if (now) {
+                    if (st == JitEntry::kQueued) g_queue.erase(std::find(g_queue.begin(), g_queue.end(), e));
+                    e->state = JitEntry::kBusy;
+                    mine.push_back(e);
+                } else if (!g_stop) {
+                    e->state = JitEntry::kQueued;
+                    g_queue.push_back(e);
+                }
+            }
+        }
+        g_busy += int(mine.size());
+        if (!now && !g_queue.empty() && !g_worker && !g_stop) {
+            g_worker = new std::thread(worker_loop);
+            std::atexit(stop_worker);
+        }
+    }
+    g_cv.notify_all();
+    if (!now) return;
+    if (!mine.empty()) compile_batch(mine);
+    // entries the worker is compiling right now
+    std::unique_lock<std::mutex> lock(g_mu);
+    g_cv.wait(lock, [&] {
+        for (JitEntry* e : entries)
+            if (e->state.load() == JitEntry::kBusy) return false;
+        return true;
+    });
+}
+
+void jit_wait() {
+    std::unique_lock<std::mutex> lock(g_mu);
+    g_cv.wait(lock, [] { return g_queue.empty() && g_busy == 0; });
+}
+
+hipError_t jit_launch(JitEntry* e, unsigned grid, hipStream_t st, v2d* psi, const PassDesc& d, const RegOp* dops) {
+    int dev = 0;
+    hipError_t err = hipGetDevice(&dev);
+    if (err != hipSuccess) return err;
+    hipFunction_t fn = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_mu);
+        auto it = e->fn.find(dev);
+        if (it == e->fn.end()) {
+            CodeObject& obj = *e->obj;
+            auto mi = obj.mod.find(dev);
+            if (mi == obj.mod.end()) {
+                hipModule_t mod = nullptr;
+                err = hipModuleLoadData(&mod, obj.code.data());
+                if (err != hipSuccess) return err;
+                mi = obj.mod.emplace(dev, mod).first;
+            }
+            err = hipModuleGetFunction(&fn, mi->second, e->name.c_str());
+            if (err != hipSuccess) return err;
+            e->fn[dev] = fn;
+        } else {
+            fn = it->second;
+        }
+    }
+    PassDesc dd = d;
+    const double* M = reinterpret_cast<const double*>(dops);
+    void* args[] = {&psi, &dd, &M};
+    err = hipModuleLaunchKernel(fn, grid, 1, 1, 256, 1, 1, 0, st, args, nullptr);
+    if (err == hipSuccess) ++g_launches;
+    return err;
+}
+
+void jit_stats(uint64_t* compiled, uint64_t* launches, uint64_t* failures) {
+    if (compiled) *compiled = g_compiled.load();
+    if (launches) *launches = g_launches.load();
+    if (failures) *failures = g_failures.load();
+}
+
+}  // namespace llsv

@@ -19,6 +19,7 @@ struct PassDesc {
     int n, b, L, nops;
     int tq[kMaxTileBits];
     int pos[kMaxTileBits];
+    int rq[4];  // register-resident passes: register bits of the final layout (ascending)
 };
 
 // One op of a pass, in tile-local coordinates.
@@ -29,15 +30,7 @@ enum PassOpKind : int {
     P_CNOT = 3,    // a = local control, b = local target
     P_MCZ = 4,     // negate where (l & lmask) == lmask, if (tile base & omask) == omask
     P_DENSE = 5,   // a = k, q[0..k) local bits, moff = offset (in v2d) of the matrix
-    // register-resident variant (b = 12) only:
-    P_LAYOUT = 6,  // q[0..4) = the 4 local bits held in registers from now on (ascending)
-    R_1Q = 7,      // a = register bit (0..3), m = 2x2
-    R_CNOT = 8,    // a = local control bit, b = register bit of the target
 };
-
-// Register-resident tiles: 2^12 amplitudes per workgroup of 256 threads, 16 per thread.
-constexpr int kRegTileBits = 12;
-constexpr int kRegBits = 4;
 
 struct PassOp {
     int kind;
@@ -47,6 +40,39 @@ struct PassOp {
     uint64_t lmask, omask;
     double m[8];
     uint32_t dep[32];  // P_DENSE: local index of operator column c, precomputed on the host
+};
+
+// Register-resident tiles: 2^12 amplitudes per workgroup of 256 threads, 16 per thread.
+// Register j of a lane holds local index lt | R(j), where lt is the lane's share of the
+// local index and R(j) deposits the 4 bits of j at the current register bits.
+constexpr int kRegTileBits = 12;
+constexpr int kRegBits = 4;
+
+// One op of a register-resident pass.  Everything that depends on the current register
+// layout (which registers an op touches, which lanes, the LDS slots of a layout change) is
+// resolved on the host, so the kernel does only the arithmetic and the data movement.
+enum RegOpKind : int {
+    RO_1Q = 0,      // a = register bit of the target, m = 2x2
+    RO_CNOT = 1,    // a = register bit of the target; registers j (bit a clear) in rmask swap
+                    // with j | 2^a on lanes whose local index bits include tmask
+    RO_DIAG = 2,    // d1 = m[6..7] on registers in rmask (tmask = 0) or on lanes holding tmask,
+                    // d0 = m[0..1] elsewhere; a = 1 when d0 == 1 (those amplitudes are left alone)
+    RO_MCZ = 3,     // negate registers in rmask on lanes holding tmask, if (base & omask) == omask
+    RO_SCALE = 4,   // per-tile scalar: d0/d1 selected by global qubit a (outside the tile)
+    RO_LAYOUT = 5,  // q[0..4) = new register bits (ascending); off0[j] / off1[j] = LDS byte
+                    // slot of register j before / after (the lane's part is XORed in)
+    RO_DENSE = 6,   // a = k, q[0..k) = local target bits, tmask = their mask, moff = matrix;
+                    // off0[j] = LDS byte slot of register j, off1[j] = R(j), dep[c] = column c
+};
+
+struct RegOp {
+    int32_t kind, a;
+    uint32_t rmask, tmask;
+    uint64_t omask;
+    int32_t moff, q[5];
+    double m[8];
+    uint32_t off0[16], off1[16];
+    uint16_t dep[32];
 };
 
 // Per-state host/device buffers of the program path.
@@ -60,6 +86,7 @@ struct ProgramCache {
 };
 
 int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops);
+int jit_precompile(int n, const llsv_op* ops, uint64_t n_ops);
 int device_tile_bits(llsv_state* s);
 
 // The scheduler alone (host only): fills order[] with the op indices in execution order and

@@ -20,7 +20,9 @@
 
 #include "sv_dense.h"
 #include "sv_kernels.h"
+#include "sv_jit.h"
 #include "sv_program.h"
+#include "sv_regops.h"
 #include "sv_state.h"
 
 namespace llsv {
@@ -152,31 +154,19 @@ __global__ __launch_bounds__(NT) void k_pass(v2d* __restrict__ psi, PassDesc d, 
 constexpr int kPassThreads = 256;
 
 // ---------------------------------------------------------------- register-resident tiles
-// The tile (2^12 amplitudes) lives in registers, 16 per thread: 4 "register bits" S of the
+// The tile (2^12 amplitudes) lives in registers, 16 per thread: 4 "register bits" of the
 // local index select the register, the other 8 local bits are the thread id (ascending bit
-// order both ways).  A one-qubit gate or a CNOT whose target is in S is a pure register
-// update; diagonal ops (phases, CZ, MCZ) need no data movement in any layout, since each
-// thread knows the index of every amplitude it holds.  LDS is used only to change S (write
-// all, barrier, read all) and to feed dense blocks.  LDS addresses are XOR-swizzled
-// (bits 0..3 ^= bits 4..7) so that layouts whose thread bits start above bit 3 do not
-// serialise on one bank.
-__device__ __forceinline__ uint32_t swz(uint32_t l) { return l ^ ((l >> 4) & 15u); }
-
-__device__ __forceinline__ uint32_t reg_dep(int j, const uint32_t* sb) {
-    return ((j & 1) ? sb[0] : 0u) | ((j & 2) ? sb[1] : 0u) | ((j & 4) ? sb[2] : 0u) | ((j & 8) ? sb[3] : 0u);
-}
-
-__device__ __forceinline__ uint32_t thread_base(uint32_t tid, uint32_t smask) {
-    uint32_t out = 0;
-    int k = 0;
-#pragma unroll
-    for (int j = 0; j < kRegTileBits; ++j)
-        if (!((smask >> j) & 1u)) {
-            out |= ((tid >> k) & 1u) << j;
-            ++k;
-        }
-    return out;
-}
+// order both ways).  A one-qubit gate or a CNOT whose target is a register bit is a pure
+// register update; diagonal ops (phases, CZ, MCZ) need no data movement in any layout, since
+// each thread knows the index of every amplitude it holds.  LDS is used only to change the
+// register bits (write all, barrier, read all) and to feed dense blocks.  The host resolves
+// every layout-dependent quantity (see RegOp), so per op the kernel reads a few uniform
+// words and runs straight-line register code.
+//
+// LDS slots are XOR-swizzled, slot(l) = l ^ ((l >> 4) & 15): a layout change whose register
+// bits include bits 4..7 would otherwise put all 64 lanes of a wavefront on one bank.  The
+// swizzle is XOR-linear, so slot(lt | R(j)) = slot(lt) ^ slot(R(j)): one XOR per access.
+LLSV_REGOPS_SOURCE
 
 __device__ __forceinline__ uint64_t local_to_global(const PassDesc& d, int bit) {
     return bit < d.L ? (uint64_t(1) << bit) : (uint64_t(1) << d.tq[bit]);
@@ -186,37 +176,48 @@ template <int K>
 __device__ __forceinline__ void reg_1q(v2d* a, const double* m) {
 #pragma unroll
     for (int j = 0; j < 16; ++j)
-        if (!(j & (1 << K))) {
-            const v2d x = a[j], y = a[j | (1 << K)];
-            a[j] = row2(m, x, y);
-            a[j | (1 << K)] = row2(m + 4, x, y);
-        }
+        if (!(j & (1 << K))) r_pu(a[j], a[j | (1 << K)], m);
 }
 
+// CNOT with the control on a register bit: registers in rmask swap with their partner.
+// Written as selects on a uniform condition, so the register array keeps one assignment
+// across all op kinds (a branchy swap makes the compiler copy the whole array every op).
 template <int K>
-__device__ __forceinline__ void reg_cnot(v2d* a, uint32_t lt, const uint32_t* sb, uint32_t cbit) {
+__device__ __forceinline__ void reg_swap(v2d* a, uint32_t rmask) {
 #pragma unroll
     for (int j = 0; j < 16; ++j)
         if (!(j & (1 << K))) {
-            if ((lt | reg_dep(j, sb)) & cbit) {
-                const v2d x = a[j];
-                a[j] = a[j | (1 << K)];
-                a[j | (1 << K)] = x;
-            }
+            const bool c = (rmask >> j) & 1u;
+            const v2d x = a[j], y = a[j | (1 << K)];
+            a[j] = c ? y : x;
+            a[j | (1 << K)] = c ? x : y;
+        }
+}
+
+// CNOT with the control on a thread bit: the swap happens on lanes with tok set.
+template <int K>
+__device__ __forceinline__ void reg_cswap(v2d* a, bool tok) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+        if (!(j & (1 << K))) {
+            const v2d x = a[j], y = a[j | (1 << K)];
+            a[j] = tok ? y : x;
+            a[j | (1 << K)] = tok ? x : y;
         }
 }
 
 template <bool DENSE>
-__global__ __launch_bounds__(256) void k_pass_reg(v2d* __restrict__ psi, PassDesc d, const PassOp* __restrict__ ops,
+__global__ __launch_bounds__(256) void k_pass_reg(v2d* __restrict__ psi, PassDesc d, const RegOp* __restrict__ ops,
                                                   const v2d* __restrict__ mats) {
     extern __shared__ v2d tile[];
+    char* const lds = reinterpret_cast<char*>(tile);
     const uint32_t tid = threadIdx.x;
     uint64_t base = blockIdx.x;
     for (int j = 0; j < kRegTileBits; ++j) base = insert0(base, d.pos[j]);
 
-    // initial layout: S = local bits 8..11, thread bits 0..7 (coalesced runs of 2^L)
-    uint32_t sb[4] = {1u << 8, 1u << 9, 1u << 10, 1u << 11};
-    uint32_t lt = tid;
+    // initial layout: register bits = local bits 8..11, thread bits = 0..7 (coalesced runs of 2^L)
+    uint32_t lt = tid;             // this lane's share of the local index
+    uint32_t sl = r_swz(tid) << 4;  // its LDS byte slot
     v2d a[16];
     {
         uint64_t gt = 0;
@@ -232,75 +233,86 @@ __global__ __launch_bounds__(256) void k_pass_reg(v2d* __restrict__ psi, PassDes
         }
     }
 
+    // Every op kind (and every target register bit) is its own if-then region with no else:
+    // the register array then merges through simple two-way joins, which the compiler keeps
+    // in one register assignment (an if-else chain or a switch makes it copy the array).
     double sr = 1.0, si = 0.0;
     for (int i = 0; i < d.nops; ++i) {
-        const PassOp& op = ops[i];
+        const RegOp& op = ops[i];
         const int kind = op.kind;
-        if (kind == R_1Q) {
-            switch (op.a) {
-                case 0: reg_1q<0>(a, op.m); break;
-                case 1: reg_1q<1>(a, op.m); break;
-                case 2: reg_1q<2>(a, op.m); break;
-                default: reg_1q<3>(a, op.m); break;
-            }
-        } else if (kind == R_CNOT) {
-            const uint32_t cbit = 1u << op.a;
-            switch (op.b) {
-                case 0: reg_cnot<0>(a, lt, sb, cbit); break;
-                case 1: reg_cnot<1>(a, lt, sb, cbit); break;
-                case 2: reg_cnot<2>(a, lt, sb, cbit); break;
-                default: reg_cnot<3>(a, lt, sb, cbit); break;
-            }
-        } else if (kind == P_DIAG) {
-            const uint32_t bit = 1u << op.a;
+        if (kind == RO_1Q) {
+            const int k = op.a;
+            if (k == 0) reg_1q<0>(a, op.m);
+            if (k == 1) reg_1q<1>(a, op.m);
+            if (k == 2) reg_1q<2>(a, op.m);
+            if (k == 3) reg_1q<3>(a, op.m);
+        }
+        if (kind == RO_MCZ && (base & op.omask) == op.omask) {
+            const uint32_t sgn = ((lt & op.tmask) == op.tmask) ? 0x80000000u : 0u;
+            const uint32_t rm = op.rmask;
 #pragma unroll
-            for (int j = 0; j < 16; ++j)
-                a[j] = ((lt | reg_dep(j, sb)) & bit) ? cmul(op.m[6], op.m[7], a[j]) : cmul(op.m[0], op.m[1], a[j]);
-        } else if (kind == P_MCZ) {
-            if ((base & op.omask) == op.omask) {
-                const uint32_t lm = uint32_t(op.lmask);
+            for (int j = 0; j < 16; ++j) a[j] = r_ng(a[j], ((rm >> j) & 1u) ? sgn : 0u);
+        }
+        if (kind == RO_LAYOUT) {
 #pragma unroll
-                for (int j = 0; j < 16; ++j)
-                    if (((lt | reg_dep(j, sb)) & lm) == lm) a[j] = -a[j];
+            for (int j = 0; j < 16; ++j) *reinterpret_cast<v2d*>(lds + (sl ^ op.off0[j])) = a[j];
+            __syncthreads();
+            uint32_t x = tid;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) x = r_ins0(x, op.q[t]);
+            lt = x;
+            sl = r_swz(x) << 4;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) a[j] = *reinterpret_cast<const v2d*>(lds + (sl ^ op.off1[j]));
+            __syncthreads();
+        }
+        if (kind == RO_CNOT) {
+            // control on a register bit (tmask = 0): registers in rmask swap on every lane;
+            // on a thread bit: every pair swaps on the lanes holding that bit
+            const int k = op.a;
+            const uint32_t rm = op.rmask;
+            if (op.tmask == 0) {
+                if (k == 0) reg_swap<0>(a, rm);
+                if (k == 1) reg_swap<1>(a, rm);
+                if (k == 2) reg_swap<2>(a, rm);
+                if (k == 3) reg_swap<3>(a, rm);
+            } else {
+                const bool tok = (lt & op.tmask) != 0;
+                if (k == 0) reg_cswap<0>(a, tok);
+                if (k == 1) reg_cswap<1>(a, tok);
+                if (k == 2) reg_cswap<2>(a, tok);
+                if (k == 3) reg_cswap<3>(a, tok);
             }
-        } else if (kind == P_SCALE) {
+        }
+        if (kind == RO_DIAG) {
+            const uint32_t rm = op.rmask;
+            const bool lane_hi = (lt & op.tmask) != 0;  // tmask = 0 when the bit is a register bit
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const bool hi = lane_hi || ((rm >> j) & 1u);
+                r_sc(a[j], hi ? op.m[6] : op.m[0], hi ? op.m[7] : op.m[1]);
+            }
+        }
+        if (kind == RO_SCALE) {
             const bool hi = (base >> op.a) & 1;
-            const double dr = hi ? op.m[6] : op.m[0], di = hi ? op.m[7] : op.m[1];
-            const double nr = sr * dr - si * di;
-            si = sr * di + si * dr;
-            sr = nr;
-        } else if (kind == P_LAYOUT) {
+            r_phase(sr, si, hi ? op.m[6] : op.m[0], hi ? op.m[7] : op.m[1]);
+        }
+        if (DENSE && kind == RO_DENSE) {
 #pragma unroll
-            for (int j = 0; j < 16; ++j) tile[swz(lt | reg_dep(j, sb))] = a[j];
-            __syncthreads();
-            uint32_t smask = 0;
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                sb[t] = 1u << op.q[t];
-                smask |= sb[t];
-            }
-            lt = thread_base(tid, smask);
-#pragma unroll
-            for (int j = 0; j < 16; ++j) a[j] = tile[swz(lt | reg_dep(j, sb))];
-            __syncthreads();
-        } else if (DENSE && kind == P_DENSE) {
-#pragma unroll
-            for (int j = 0; j < 16; ++j) tile[swz(lt | reg_dep(j, sb))] = a[j];
+            for (int j = 0; j < 16; ++j) *reinterpret_cast<v2d*>(lds + (sl ^ op.off0[j])) = a[j];
             __syncthreads();
             const int k = op.a, D = 1 << k;
-            uint32_t tmask = 0;
-            for (int t = 0; t < k; ++t) tmask |= 1u << op.q[t];
             const v2d* U = mats + op.moff;
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
-                const uint32_t o = lt | reg_dep(j, sb);
+                const uint32_t o = lt | op.off1[j];
                 uint32_t row = 0;
                 for (int t = 0; t < k; ++t) row |= ((o >> op.q[t]) & 1u) << t;
-                const uint32_t gb = o & ~tmask;
+                const uint32_t gb = o & ~op.tmask;
                 const v2d* ur = U + row * D;
                 v2d acc = {0.0, 0.0};
                 for (int c = 0; c < D; ++c) {
-                    const v2d u = ur[c], x = tile[swz(gb | op.dep[c])];
+                    const v2d u = ur[c], x = tile[r_swz(gb | op.dep[c])];
                     acc.x += u.x * x.x - u.y * x.y;
                     acc.y += u.x * x.y + u.y * x.x;
                 }
@@ -316,12 +328,15 @@ __global__ __launch_bounds__(256) void k_pass_reg(v2d* __restrict__ psi, PassDes
         if ((lt >> j) & 1u) gt |= local_to_global(d, j);
     uint64_t gs[4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) gs[t] = local_to_global(d, 31 - __builtin_clz(sb[t]));
-    const bool unit = (sr == 1.0 && si == 0.0);
+    for (int t = 0; t < 4; ++t) gs[t] = local_to_global(d, d.rq[t]);
+    if (sr != 1.0 || si != 0.0) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) r_sc(a[j], sr, si);
+    }
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
         const uint64_t go = ((j & 1) ? gs[0] : 0) | ((j & 2) ? gs[1] : 0) | ((j & 4) ? gs[2] : 0) | ((j & 8) ? gs[3] : 0);
-        st_nt(psi + (base | gt | go), unit ? a[j] : cmul(sr, si, a[j]));
+        st_nt(psi + (base | gt | go), a[j]);
     }
 }
 
@@ -506,12 +521,40 @@ void merge_1q(PassOp* prev, const PassOp& next) {
     prev->kind = (prev->kind == P_DIAG && next.kind == P_DIAG) ? P_DIAG : P_1Q;
 }
 
+// Local index bits of register j when the register bits are S (4 bits of a 12-bit mask):
+// bit t of j lands on the t-th lowest bit of S.
+uint32_t reg_local(uint32_t S, int j) {
+    uint32_t o = 0;
+    int t = 0;
+    for (int bit = 0; bit < kRegTileBits; ++bit)
+        if ((S >> bit) & 1u) {
+            if ((j >> t) & 1) o |= 1u << bit;
+            ++t;
+        }
+    return o;
+}
+
+// LDS byte slot of the register part of a local index (the swizzle of the kernel, x 16 B).
+uint32_t reg_slot(uint32_t S, int j) {
+    const uint32_t l = reg_local(S, j);
+    return (l ^ ((l >> 4) & 15u)) << 4;
+}
+
+// Registers whose register part of the local index contains every bit of m (m within S).
+uint32_t regs_with(uint32_t S, uint32_t m) {
+    uint32_t r = 0;
+    for (int j = 0; j < 16; ++j)
+        if ((reg_local(S, j) & m) == m) r |= 1u << j;
+    return r;
+}
+
 // Turns a pass's ops (tile-local coordinates) into the register-resident program: a layout
 // change is inserted before every one-qubit gate or CNOT whose target bit is not held in
 // registers; the new register set takes that bit, then the target bits of the next ops (so
 // one change serves several gates), then bits of the current set.  The pass ends in a layout
-// whose thread bits include local bits 0..L-1, so the stores write whole lines.
-std::vector<PassOp> to_register_program(const std::vector<PassOp>& in, int L) {
+// whose thread bits include local bits 0..L-1, so the stores write whole lines; its register
+// bits go to rq.
+std::vector<RegOp> to_register_program(const std::vector<PassOp>& in, int L, int rq[4]) {
     constexpr uint32_t kStart = 0xF00u;  // local bits 8..11
     const uint32_t low = (1u << L) - 1;
     auto need_of = [](const PassOp& po) -> uint32_t {
@@ -520,15 +563,19 @@ std::vector<PassOp> to_register_program(const std::vector<PassOp>& in, int L) {
         return 0u;
     };
     auto rank = [](uint32_t set, int bit) { return __builtin_popcount(set & ((1u << bit) - 1)); };
-    auto layout = [](uint32_t set) {
-        PassOp lay{};
-        lay.kind = P_LAYOUT;
+    auto layout = [](uint32_t from, uint32_t to) {
+        RegOp lay{};
+        lay.kind = RO_LAYOUT;
         int t = 0;
         for (int bit = 0; bit < kRegTileBits; ++bit)
-            if ((set >> bit) & 1u) lay.q[t++] = bit;
+            if ((to >> bit) & 1u) lay.q[t++] = bit;
+        for (int j = 0; j < 16; ++j) {
+            lay.off0[j] = reg_slot(from, j);
+            lay.off1[j] = reg_slot(to, j);
+        }
         return lay;
     };
-    std::vector<PassOp> out;
+    std::vector<RegOp> out;
     out.reserve(in.size() + 8);
     uint32_t S = kStart;
     for (size_t i = 0; i < in.size(); ++i) {
@@ -542,67 +589,135 @@ std::vector<PassOp> to_register_program(const std::vector<PassOp>& in, int L) {
             for (int bit = kRegTileBits - 1; bit >= 0 && __builtin_popcount(ns) < kRegBits; --bit)
                 if ((S >> bit) & 1u) ns |= 1u << bit;
             for (int bit = kRegTileBits - 1; bit >= 0 && __builtin_popcount(ns) < kRegBits; --bit) ns |= 1u << bit;
-            out.push_back(layout(ns));
+            out.push_back(layout(S, ns));
             S = ns;
         }
-        PassOp po = in[i];
-        if (po.kind == P_1Q) {
-            po.kind = R_1Q;
-            po.a = rank(S, in[i].a);
-        } else if (po.kind == P_CNOT) {
-            po.kind = R_CNOT;
-            po.a = in[i].a;
-            po.b = rank(S, in[i].b);
+        const PassOp& po = in[i];
+        RegOp ro{};
+        std::memcpy(ro.m, po.m, sizeof ro.m);
+        switch (po.kind) {
+            case P_1Q:
+                ro.kind = RO_1Q;
+                ro.a = rank(S, po.a);
+                break;
+            case P_CNOT: {
+                ro.kind = RO_CNOT;
+                ro.a = rank(S, po.b);
+                const uint32_t cb = 1u << po.a;
+                if (S & cb) {
+                    ro.rmask = regs_with(S, cb);
+                } else {
+                    ro.rmask = 0xFFFFu;
+                    ro.tmask = cb;
+                }
+                break;
+            }
+            case P_DIAG: {
+                ro.kind = RO_DIAG;
+                const uint32_t bit = 1u << po.a;
+                if (S & bit) {
+                    ro.rmask = regs_with(S, bit);
+                    ro.a = (po.m[0] == 1.0 && po.m[1] == 0.0) ? 1 : 0;
+                } else {
+                    ro.tmask = bit;
+                }
+                break;
+            }
+            case P_MCZ: {
+                ro.kind = RO_MCZ;
+                const uint32_t lm = uint32_t(po.lmask);
+                ro.rmask = regs_with(S, lm & S);
+                ro.tmask = lm & ~S;
+                ro.omask = po.omask;
+                break;
+            }
+            case P_SCALE:
+                ro.kind = RO_SCALE;
+                ro.a = po.a;
+                break;
+            case P_DENSE: {
+                ro.kind = RO_DENSE;
+                ro.a = po.a;
+                ro.moff = po.moff;
+                for (int t = 0; t < po.a; ++t) {
+                    ro.q[t] = po.q[t];
+                    ro.tmask |= 1u << po.q[t];
+                }
+                for (int c = 0; c < (1 << po.a); ++c) ro.dep[c] = uint16_t(po.dep[c]);
+                for (int j = 0; j < 16; ++j) {
+                    ro.off0[j] = reg_slot(S, j);
+                    ro.off1[j] = reg_local(S, j);
+                }
+                break;
+            }
         }
-        out.push_back(po);
+        out.push_back(ro);
     }
-    if (S & low) out.push_back(layout(kStart));
+    if (S & low) {
+        out.push_back(layout(S, kStart));
+        S = kStart;
+    }
+    int t = 0;
+    for (int bit = 0; bit < kRegTileBits; ++bit)
+        if ((S >> bit) & 1u) rq[t++] = bit;
     return out;
 }
 
 }  // namespace
 
-int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
-    const int n = s->n;
+namespace {
+
+// One kernel launch of a program: a streamed single op, a lone dense block on the matrix cores,
+// or a pass (LDS tile or register tile, interpreted or compiled).
+struct Launch {
+    bool streamed = false;
+    bool dense = false;
+    bool mfma = false;  // a lone dense block of 4-5 qubits: the FP64-MFMA kernel
+    uint64_t op = 0;
+    PassDesc d{};
+    DenseTile dt{};
+    size_t first = 0;  // first op of the pass in ops (LDS tiles) or rops (register tiles)
+    size_t moff = 0;
+};
+
+struct BuiltProgram {
+    int b = 0;
+    bool reg_tiles = false;
+    std::vector<Launch> launches;
+    std::vector<PassOp> ops;
+    std::vector<RegOp> rops;
+    std::vector<const llsv_op*> dense_src;  // dense ops in upload order
+    size_t mat_v2d = 0;
+};
+
+// Host half of llsv_apply_program: validation, the pass schedule and every pass's op list.
+int build_program(int n, const llsv_op* ops, uint64_t n_ops, int tile_bits, BuiltProgram* out) {
     std::vector<Checked> info(n_ops);
     for (uint64_t i = 0; i < n_ops; ++i)
         if (int rc = check_op(n, ops[i], i, &info[i])) return rc;
 
-    const int b = std::min(n, device_tile_bits(s));
+    const int b = std::min(n, tile_bits);
     const int L = std::min(kLowBits, b);
     const bool reg_tiles = (b == kRegTileBits);
+    out->b = b;
+    out->reg_tiles = reg_tiles;
     const std::vector<PlannedPass> passes = plan(n, ops, info, b, L);
-
-    struct Launch {
-        bool streamed;
-        bool dense;
-        bool mfma;  // a lone dense block of 4-5 qubits: the FP64-MFMA kernel
-        uint64_t op;
-        PassDesc d;
-        DenseTile dt;
-        size_t first;
-        size_t moff;
-    };
-    std::vector<Launch> launches;
-    launches.reserve(passes.size());
-    std::vector<PassOp> all_ops;
-    std::vector<const llsv_op*> dense_src;  // dense ops in upload order
-    size_t mat_v2d = 0;
+    out->launches.reserve(passes.size());
     for (const PlannedPass& p : passes) {
         Launch la{};
         if (p.ops.size() == 1 && info[p.ops[0]].single && n > b) {
             la.streamed = true;
             la.op = p.ops[0];
-            launches.push_back(la);
+            out->launches.push_back(la);
             continue;
         }
         if (p.ops.size() == 1 && ops[p.ops[0]].kind == LLSV_OP_DENSE &&
             make_dense_tile(n, ops[p.ops[0]].nq, ops[p.ops[0]].q, &la.d, &la.dt)) {
             la.mfma = true;
-            la.moff = mat_v2d;
-            dense_src.push_back(&ops[p.ops[0]]);
-            mat_v2d += size_t(1) << (2 * ops[p.ops[0]].nq);
-            launches.push_back(la);
+            la.moff = out->mat_v2d;
+            out->dense_src.push_back(&ops[p.ops[0]]);
+            out->mat_v2d += size_t(1) << (2 * ops[p.ops[0]].nq);
+            out->launches.push_back(la);
             continue;
         }
         PassDesc& d = la.d;
@@ -666,15 +781,15 @@ int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
                     po.kind = P_DENSE;
                     po.a = op.nq;
                     for (int t = 0; t < op.nq; ++t) po.q[t] = local_of[op.q[t]];
-                    po.moff = int(mat_v2d);
+                    po.moff = int(out->mat_v2d);
                     for (int c = 0; c < (1 << op.nq); ++c) {
                         uint32_t o = 0;
                         for (int t = 0; t < op.nq; ++t)
                             if ((c >> t) & 1) o |= 1u << po.q[t];
                         po.dep[c] = o;
                     }
-                    dense_src.push_back(&op);
-                    mat_v2d += size_t(1) << (2 * op.nq);
+                    out->dense_src.push_back(&op);
+                    out->mat_v2d += size_t(1) << (2 * op.nq);
                     break;
                 }
             }
@@ -696,22 +811,43 @@ int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
             }
             pops.push_back(po);
         }
-        if (reg_tiles) pops = to_register_program(pops, L);
-        d.nops = int(pops.size());
-        la.first = all_ops.size();
-        all_ops.insert(all_ops.end(), pops.begin(), pops.end());
-        launches.push_back(la);
+        if (reg_tiles) {
+            const std::vector<RegOp> rops = to_register_program(pops, L, d.rq);
+            d.nops = int(rops.size());
+            la.first = out->rops.size();
+            out->rops.insert(out->rops.end(), rops.begin(), rops.end());
+        } else {
+            d.nops = int(pops.size());
+            la.first = out->ops.size();
+            out->ops.insert(out->ops.end(), pops.begin(), pops.end());
+        }
+        out->launches.push_back(la);
     }
 
-    const size_t ops_bytes = all_ops.size() * sizeof(PassOp);
-    const size_t mat_off_bytes = (ops_bytes + 255) & ~size_t(255);
-    const size_t total = mat_off_bytes + mat_v2d * sizeof(v2d);
-    if (all_ops.size() || mat_v2d) {
+    return LLSV_OK;
+}
+
+}  // namespace
+
+int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
+    const int n = s->n;
+    BuiltProgram bp;
+    if (int rc = build_program(n, ops, n_ops, device_tile_bits(s), &bp)) return rc;
+    const int b = bp.b;
+    const bool reg_tiles = bp.reg_tiles;
+
+    const size_t ops_bytes = bp.ops.size() * sizeof(PassOp);
+    const size_t rops_off_bytes = (ops_bytes + 255) & ~size_t(255);
+    const size_t rops_bytes = bp.rops.size() * sizeof(RegOp);
+    const size_t mat_off_bytes = (rops_off_bytes + rops_bytes + 255) & ~size_t(255);
+    const size_t total = mat_off_bytes + bp.mat_v2d * sizeof(v2d);
+    if (bp.ops.size() || bp.rops.size() || bp.mat_v2d) {
         if (int rc = ensure_capacity(s, total)) return rc;
-        std::memcpy(s->program->host, all_ops.data(), ops_bytes);
+        std::memcpy(s->program->host, bp.ops.data(), ops_bytes);
+        std::memcpy(s->program->host + rops_off_bytes, bp.rops.data(), rops_bytes);
         v2d* hmats = reinterpret_cast<v2d*>(s->program->host + mat_off_bytes);
         size_t mi = 0;
-        for (const llsv_op* op : dense_src) {
+        for (const llsv_op* op : bp.dense_src) {
             const size_t cnt = size_t(1) << (2 * op->nq);
             std::memcpy(hmats + mi, op->dense, cnt * sizeof(v2d));
             mi += cnt;
@@ -721,9 +857,25 @@ int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
         if (e != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_apply_program: upload: %s", hipGetErrorString(e));
     }
     const PassOp* dops = reinterpret_cast<const PassOp*>(s->program->dev);
+    const RegOp* drops = reinterpret_cast<const RegOp*>(s->program->dev + rops_off_bytes);
     const v2d* dmats = reinterpret_cast<const v2d*>(s->program->dev + mat_off_bytes);
     const size_t lds = size_t(16) << b;
-    for (const Launch& la : launches) {
+
+    // compiled pass kernels: look up (and request) every eligible pass first, so the uncompiled
+    // ones go to hipRTC as one batch
+    std::vector<JitEntry*> jit(bp.launches.size(), nullptr);
+    if (reg_tiles) {
+        std::vector<JitEntry*> want;
+        for (size_t i = 0; i < bp.launches.size(); ++i) {
+            const Launch& la = bp.launches[i];
+            if (la.streamed || la.mfma || !jit_wanted(n, la.d.nops, la.dense)) continue;
+            jit[i] = jit_entry(jit_source(bp.rops.data() + la.first, la.d.nops));
+            if (!jit_ready(jit[i])) want.push_back(jit[i]);
+        }
+        if (!want.empty()) jit_request(want, jit_mode() == kJitAlways);
+    }
+    for (size_t li = 0; li < bp.launches.size(); ++li) {
+        const Launch& la = bp.launches[li];
         hipError_t e = hipSuccess;
         if (la.mfma) {
             e = launch_dense_mfma(s->stream, s->amps, la.d, la.dt, dmats + la.moff);
@@ -748,26 +900,67 @@ int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
         } else {
             const uint64_t tiles = uint64_t(1) << (n - b);
             const dim3 grid{unsigned(tiles), 1, 1}, block{unsigned(kPassThreads), 1, 1};
-            if (reg_tiles) {
+            if (jit_ready(jit[li])) {
+                // a pass whose kernel is not ready (compiling, or failed to compile) is interpreted
+                e = jit_launch(jit[li], unsigned(tiles), s->stream, s->amps, la.d, drops + la.first);
+            } else if (reg_tiles) {
                 if (la.dense)
-                    hipLaunchKernelGGL((k_pass_reg<true>), grid, block, lds, s->stream, s->amps, la.d, dops + la.first, dmats);
+                    hipLaunchKernelGGL((k_pass_reg<true>), grid, block, lds, s->stream, s->amps, la.d, drops + la.first, dmats);
                 else
-                    hipLaunchKernelGGL((k_pass_reg<false>), grid, block, lds, s->stream, s->amps, la.d, dops + la.first, dmats);
+                    hipLaunchKernelGGL((k_pass_reg<false>), grid, block, lds, s->stream, s->amps, la.d, drops + la.first, dmats);
+                e = hipGetLastError();
             } else if (la.dense) {
                 hipLaunchKernelGGL((k_pass<kPassThreads, true>), grid, block, lds, s->stream, s->amps, la.d,
                                    dops + la.first, dmats);
+                e = hipGetLastError();
             } else {
                 hipLaunchKernelGGL((k_pass<kPassThreads, false>), grid, block, lds, s->stream, s->amps, la.d,
                                    dops + la.first, dmats);
+                e = hipGetLastError();
             }
-            e = hipGetLastError();
         }
         if (e != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_apply_program: launch: %s", hipGetErrorString(e));
     }
     return LLSV_OK;
 }
 
+// Compiles the pass kernels a program would use on an n-qubit state with 12-bit register tiles
+// (host only); returns how many passes have a compiled kernel.
+int jit_precompile(int n, const llsv_op* ops, uint64_t n_ops) {
+    BuiltProgram bp;
+    if (int rc = build_program(n, ops, n_ops, kRegTileBits, &bp)) return rc;
+    if (!bp.reg_tiles) return 0;
+    std::vector<JitEntry*> want;
+    for (const Launch& la : bp.launches) {
+        if (la.streamed || la.mfma || !jit_wanted(n, la.d.nops, la.dense)) continue;
+        want.push_back(jit_entry(jit_source(bp.rops.data() + la.first, la.d.nops)));
+    }
+    jit_request(want, true);
+    int count = 0;
+    for (JitEntry* e : want) count += jit_ready(e) ? 1 : 0;
+    return count;
+}
+
 }  // namespace llsv
+
+extern "C" int llsv_set_program_jit(int mode) {
+    if (mode < llsv::kJitOff || mode > llsv::kJitAlways)
+        return llsv::fail(LLSV_ERR_ARG, "llsv_set_program_jit: mode must be 0, 1 or 2");
+    return llsv::set_jit_mode(mode);
+}
+
+extern "C" void llsv_program_jit_stats(uint64_t* compiled, uint64_t* launches, uint64_t* failures) {
+    llsv::jit_stats(compiled, launches, failures);
+}
+
+extern "C" int llsv_program_jit_precompile(int n_qubits, const llsv_op* ops, uint64_t n_ops) {
+    using namespace llsv;
+    if (n_qubits < 1 || n_qubits > LLSV_MAX_QUBITS) return fail(LLSV_ERR_ARG, "llsv_program_jit_precompile: bad qubit count");
+    if (n_ops && !ops) return fail(LLSV_ERR_ARG, "llsv_program_jit_precompile: ops is null");
+    return jit_precompile(n_qubits, ops, n_ops);
+}
+
+extern "C" void llsv_program_jit_wait(void) { llsv::jit_wait(); }
 
 extern "C" int llsv_program_tile_bits(llsv_state* s) {
     if (!s) return llsv::fail(LLSV_ERR_HANDLE, "llsv_program_tile_bits: null state handle");

@@ -38,6 +38,10 @@ SIGNATURES = {
     "llsv_apply_program": (_I, [_P, _P, _U64]),
     "llsv_plan_passes": (_I, [_I, _P, _U64, _I, ctypes.POINTER(_U64), _P, _P]),
     "llsv_program_tile_bits": (_I, [_P]),
+    "llsv_set_program_jit": (_I, [_I]),
+    "llsv_program_jit_stats": (None, [ctypes.POINTER(_U64), ctypes.POINTER(_U64), ctypes.POINTER(_U64)]),
+    "llsv_program_jit_precompile": (_I, [_I, _P, _U64]),
+    "llsv_program_jit_wait": (None, []),
     "llsv_norm2": (_I, [_P, ctypes.POINTER(_D)]),
     "llsv_expect_z": (_I, [_P, _I, ctypes.POINTER(_D)]),
     "llsv_probabilities": (_I, [_P, _P, _U64]),
