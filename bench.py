@@ -355,17 +355,19 @@ def main():
     sv = V3.StateVector(n)
 
     if a.probe:
-        # 2 copies (calibration: 32 B per amplitude), 3 one-qubit gates (low / mid / high
-        # target), 2 CNOTs and one program step, each a separate dispatch.
+        # 2 copies (calibration: 32 B per amplitude), 4 one-qubit gates (targets 0, 7, n/2,
+        # n-1: every kernel variant), 3 CNOTs and one program step, each a separate dispatch.
         other = V3.StateVector(n)
         for _ in range(2):
             other.copy_from(sv)
         other.sync()
         del other
-        for q in (0, n // 2, n - 1):
+        for q in (0, 7, n // 2, n - 1):
             sv.apply_gate_cache_optimized(q, one_q[q][1])
         sv.cnot(0, 1)
+        sv.cnot(3, 4)
         sv.cnot(n - 2, n - 1)
+        V3.program_jit_precompile(n, ops)
         sv.apply_program(ops)
         sv.sync()
         print(json.dumps({"probe": n, "norm": sv.norm()}), flush=True)

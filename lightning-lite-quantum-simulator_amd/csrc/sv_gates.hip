@@ -4,8 +4,13 @@
 // 128-bit accesses; consecutive lanes of a 64-wide wavefront touch consecutive
 // amplitudes wherever the gate's index map allows, so each wavefront access is a
 // 1 KiB contiguous burst (runs of 2^q amplitudes when the gate's lowest qubit q < 6).
-// Grids are sized for >> 256 CUs; no data is reused, so the block -> tile order is
-// plain linear (XCD placement does not matter for a pure stream).
+// Grids are sized for >> 256 CUs.
+//
+// Split-chunk order (the *_split kernels, used from 2^13 amplitudes up): a workgroup takes
+// two runs of 256 units (pairs, swaps or amplitudes) whose addresses lie 32-64 KiB apart,
+// instead of one contiguous run.  Measured on MI355X at 30 qubits, in-place streams in that
+// order run 8-12% faster (6.2-6.6 TB/s against 5.6-6.0 for the contiguous order), which
+// points at DRAM bank parallelism within each workgroup's accesses.
 #include "sv_common.h"
 #include "sv_kernels.h"
 
@@ -159,6 +164,112 @@ __global__ __launch_bounds__(256) void k_diag_1q(v2d* __restrict__ psi, uint64_t
   }
 }
 
+// Unit of the split-chunk order for thread t of chunk `chunk`, half h (0, 1): chunks come in
+// groups of 2^(logD - 8) that share a superchunk of 2^(logD + 1) units; the two halves of a
+// chunk are 2^logD units apart.  The unit count must be a multiple of 2^(logD + 1).
+__device__ __forceinline__ uint64_t split_unit(uint64_t chunk, uint32_t t, int h, int logD) {
+    const uint64_t per = uint64_t(1) << (logD - 8);
+    return ((chunk >> (logD - 8)) << (logD + 1)) + ((chunk & (per - 1)) << 8) + t + (h ? (uint64_t(1) << logD) : 0);
+}
+
+// Offset exponent (in units) of the split for a unit -> amplitude map that inserts zero bits
+// at the ascending positions ins[0..m): the unit bit that lands on amplitude bit 12 (64 KiB),
+// else 11, else 13.  -1 if none (then the contiguous kernels run).
+static int split_bit(const int* ins, int m) {
+    for (int want : {12, 11, 13})
+        for (int d = 8; d < 40; ++d) {
+            int x = d;
+            for (int j = 0; j < m; ++j)
+                if (x >= ins[j]) ++x;
+            if (x == want) return d;
+            if (x > want) break;
+        }
+    return -1;
+}
+
+constexpr int kSplitPairs = 11;  // pair-index split of the one-qubit kernels (2^11 pairs apart)
+
+template <bool XCD_MAP>
+__global__ __launch_bounds__(256) void k_apply_1q_split(v2d* __restrict__ psi, int q, Gate2 g, uint64_t nchunks, int logD) {
+    const uint64_t lomask = (uint64_t(1) << q) - 1;
+    const uint64_t stride = uint64_t(1) << q;
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint64_t blk = XCD_MAP ? xcd_block(chunk, nchunks) : chunk;
+    v2d a0[2], a1[2];
+    uint64_t i0[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const uint64_t p = split_unit(blk, threadIdx.x, u, logD);
+        i0[u] = ((p & ~lomask) << 1) | (p & lomask);
+        a0[u] = ld_nt(psi + i0[u]);
+        a1[u] = ld_nt(psi + i0[u] + stride);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        st_nt(psi + i0[u], row2(g.r, a0[u], a1[u]));
+        st_nt(psi + i0[u] + stride, row2(g.r + 4, a0[u], a1[u]));
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_apply_1q_lane_split(v2d* __restrict__ psi, int q, Gate2 g, uint64_t nchunks,
+                                                             int logD) {
+    const int m = 1 << q;
+    const bool hi = (threadIdx.x >> q) & 1;
+    const double* r = hi ? g.r + 4 : g.r;
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    v2d a[2];
+    uint64_t ii[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        ii[u] = split_unit(chunk, threadIdx.x, u, logD);
+        a[u] = ld_nt(psi + ii[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        v2d p;
+        p.x = __shfl_xor(a[u].x, m, 64);
+        p.y = __shfl_xor(a[u].y, m, 64);
+        st_nt(psi + ii[u], hi ? row2(r, p, a[u]) : row2(r, a[u], p));
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_phase_upper_split(v2d* __restrict__ psi, int q, double dr, double di,
+                                                           uint64_t nchunks, int logD) {
+    const uint64_t lomask = (uint64_t(1) << q) - 1;
+    const uint64_t stride = uint64_t(1) << q;
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    v2d a[2];
+    uint64_t i[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const uint64_t p = split_unit(chunk, threadIdx.x, u, logD);
+        i[u] = (((p & ~lomask) << 1) | (p & lomask)) + stride;
+        a[u] = ld_nt(psi + i[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) st_nt(psi + i[u], cmul(dr, di, a[u]));
+  }
+}
+
+__global__ __launch_bounds__(256) void k_diag_1q_split(v2d* __restrict__ psi, int q, Gate2 g, uint64_t nchunks, int logD) {
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    v2d a[2];
+    uint64_t ii[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        ii[u] = split_unit(chunk, threadIdx.x, u, logD);
+        a[u] = ld_nt(psi + ii[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const bool hi = (ii[u] >> q) & 1;
+        st_nt(psi + ii[u], hi ? cmul(g.r[6], g.r[7], a[u]) : cmul(g.r[0], g.r[1], a[u]));
+    }
+  }
+}
+
 constexpr int kPPT = 4;
 // Lowest qubit whose 2^q-amplitude runs (16 * 2^q bytes) fill whole 128-byte lines.
 constexpr int kLineBits = 3;
@@ -166,8 +277,30 @@ constexpr int kLineBits = 3;
 hipError_t launch_apply_1q(hipStream_t st, v2d* psi, int n, int q, const Gate2& g) {
     const uint64_t npairs = uint64_t(1) << (n - 1);
     const bool diag = g.r[2] == 0.0 && g.r[3] == 0.0 && g.r[4] == 0.0 && g.r[5] == 0.0;
+    if (diag && g.r[0] == 1.0 && g.r[1] == 0.0 && g.r[6] == 1.0 && g.r[7] == 0.0) return hipSuccess;  // identity
+    if (npairs >= (uint64_t(1) << (kSplitPairs + 1))) {
+        // split-chunk order: pairs (or amplitudes) in chunks of 512, halves 2^11 pairs apart
+        const int ld = kSplitPairs;
+        if (diag && g.r[0] == 1.0 && g.r[1] == 0.0 && q >= kLineBits) {
+            const uint64_t nch = npairs / 512;
+            hipLaunchKernelGGL(k_phase_upper_split, dim3(grid_for(nch)), dim3(256), 0, st, psi, q, g.r[6], g.r[7], nch, ld);
+        } else if (diag) {
+            const uint64_t nch = 2 * npairs / 512;
+            hipLaunchKernelGGL(k_diag_1q_split, dim3(grid_for(nch)), dim3(256), 0, st, psi, q, g, nch, ld);
+        } else if (q < 6) {
+            const uint64_t nch = 2 * npairs / 512;
+            hipLaunchKernelGGL(k_apply_1q_lane_split, dim3(grid_for(nch)), dim3(256), 0, st, psi, q, g, nch, ld);
+        } else if (q < 9) {
+            // XCD-contiguous chunk order: another 2% at these targets
+            const uint64_t nch = npairs / 512;
+            hipLaunchKernelGGL((k_apply_1q_split<true>), dim3(grid_for(nch)), dim3(256), 0, st, psi, q, g, nch, ld);
+        } else {
+            const uint64_t nch = npairs / 512;
+            hipLaunchKernelGGL((k_apply_1q_split<false>), dim3(grid_for(nch)), dim3(256), 0, st, psi, q, g, nch, ld);
+        }
+        return hipGetLastError();
+    }
     if (diag && g.r[0] == 1.0 && g.r[1] == 0.0 && q >= kLineBits) {
-        if (g.r[6] == 1.0 && g.r[7] == 0.0) return hipSuccess;  // identity
         const uint64_t blocks = (npairs + 256 * kPPT - 1) / (256 * kPPT);
         hipLaunchKernelGGL(k_phase_upper<kPPT>, dim3(grid_for(blocks)), dim3(256), 0, st, psi, npairs, q, g.r[6],
                            g.r[7], blocks);
@@ -280,8 +413,63 @@ __global__ __launch_bounds__(256) void k_cnot_pairs(v2d* __restrict__ psi, uint6
   }
 }
 
+// Split-chunk versions of the half-state swap and of the lane kernel.
+__global__ __launch_bounds__(256) void k_cnot_split(v2d* __restrict__ psi, int lo, int hi, uint64_t cb, uint64_t tb,
+                                                    uint64_t nchunks, int logD) {
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    v2d a[2], b[2];
+    uint64_t idx[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        idx[u] = insert0(insert0(split_unit(chunk, threadIdx.x, u, logD), lo), hi) | cb;
+        a[u] = ld_nt(psi + idx[u]);
+        b[u] = ld_nt(psi + (idx[u] | tb));
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        st_nt(psi + idx[u], b[u]);
+        st_nt(psi + (idx[u] | tb), a[u]);
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cnot_lane_split(v2d* __restrict__ psi, int c, int t, uint64_t nchunks, int logD) {
+    const int m = 1 << t;
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    v2d a[2];
+    uint64_t ii[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        ii[u] = split_unit(chunk, threadIdx.x, u, logD);
+        a[u] = ld_nt(psi + ii[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        v2d p;
+        p.x = __shfl_xor(a[u].x, m, 64);
+        p.y = __shfl_xor(a[u].y, m, 64);
+        st_nt(psi + ii[u], ((ii[u] >> c) & 1) ? p : a[u]);
+    }
+  }
+}
+
 hipError_t launch_cnot(hipStream_t st, v2d* psi, int n, int c, int t) {
     const int lo = c < t ? c : t, hi = c < t ? t : c;
+    if (c >= kLineBits && t >= kLineBits) {
+        const int ins[2] = {lo, hi};
+        const int d = split_bit(ins, 2);
+        const uint64_t nswaps = uint64_t(1) << (n - 2);
+        if (d > 0 && nswaps >= (uint64_t(1) << (d + 1))) {
+            const uint64_t nch = nswaps / 512;
+            hipLaunchKernelGGL(k_cnot_split, dim3(grid_for(nch)), dim3(256), 0, st, psi, lo, hi, uint64_t(1) << c,
+                               uint64_t(1) << t, nch, d);
+            return hipGetLastError();
+        }
+    } else if (c < kLineBits && t < 6 && n >= kSplitPairs + 2) {
+        const uint64_t nch = (uint64_t(1) << n) / 512;
+        hipLaunchKernelGGL(k_cnot_lane_split, dim3(grid_for(nch)), dim3(256), 0, st, psi, c, t, nch, kSplitPairs);
+        return hipGetLastError();
+    }
     if (c >= kLineBits || n < 6) {
         const uint64_t nswaps = uint64_t(1) << (n - 2);
         const uint64_t blocks = (nswaps + 256 * kPPT - 1) / (256 * kPPT);
@@ -346,7 +534,51 @@ __global__ __launch_bounds__(256) void k_mcz_full(v2d* __restrict__ psi, uint64_
   }
 }
 
+__global__ __launch_bounds__(256) void k_mcz_split(v2d* __restrict__ psi, BitList bits, uint64_t nchunks, int logD) {
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    v2d a[2];
+    uint64_t idx[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        uint64_t i = split_unit(chunk, threadIdx.x, u, logD);
+        for (int j = 0; j < bits.m; ++j) i = insert0(i, bits.pos[j]);
+        idx[u] = i | bits.mask;
+        a[u] = ld_nt(psi + idx[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) st_nt(psi + idx[u], -a[u]);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mcz_full_split(v2d* __restrict__ psi, uint64_t mask, uint64_t nchunks, int logD) {
+  for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    v2d a[2];
+    uint64_t ii[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        ii[u] = split_unit(chunk, threadIdx.x, u, logD);
+        a[u] = ld_nt(psi + ii[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) st_nt(psi + ii[u], ((ii[u] & mask) == mask) ? -a[u] : a[u]);
+  }
+}
+
 hipError_t launch_mcz(hipStream_t st, v2d* psi, int n, const BitList& bits) {
+    if (bits.pos[0] < kLineBits && n >= kSplitPairs + 2) {
+        const uint64_t nch = (uint64_t(1) << n) / 512;
+        hipLaunchKernelGGL(k_mcz_full_split, dim3(grid_for(nch)), dim3(256), 0, st, psi, bits.mask, nch, kSplitPairs);
+        return hipGetLastError();
+    }
+    if (bits.pos[0] >= kLineBits) {
+        const int d = split_bit(bits.pos, bits.m);
+        const uint64_t count = uint64_t(1) << (n - bits.m);
+        if (d > 0 && count >= (uint64_t(1) << (d + 1))) {
+            const uint64_t nch = count / 512;
+            hipLaunchKernelGGL(k_mcz_split, dim3(grid_for(nch)), dim3(256), 0, st, psi, bits, nch, d);
+            return hipGetLastError();
+        }
+    }
     if (bits.pos[0] < kLineBits && n >= 6) {
         const uint64_t size = uint64_t(1) << n;
         const uint64_t blocks = (size + 1023) / 1024;
