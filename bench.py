@@ -198,9 +198,10 @@ def adjoint_leg(n=20, layers=10, batch=64):
 
 def trajectory_leg(n=14, count=4096, mixed=2048):
     """4096 noisy trajectories of a 14-qubit brick circuit (8 layers of H, RZ, CNOT brick; AD then
-    PD after every gate on each touched qubit) in one persistent launch; a mixed batch of 2048
-    circuits of 10..16 qubits in one launch; and an ensemble check at 8 qubits against the
-    exact density matrix (CPU oracle, as a checker only)."""
+    PD after every gate on each touched qubit) in one persistent launch -- one-shot with the
+    default launch, then as an autotuned plan replayed as a hipGraph (the reported rate); a mixed
+    batch of 2048 circuits of 10..16 qubits likewise; and an ensemble check at 8 qubits against
+    the exact density matrix (CPU oracle, as a checker only)."""
     from llsv import circuits as C, trajectory as LT
 
     gates, gamma, lam = C.noisy_brick(0, n, 8)
@@ -209,13 +210,34 @@ def trajectory_leg(n=14, count=4096, mixed=2048):
     z, j, _ = LT.run_trajectories(n, gates, gamma, lam, seed=1, n_traj=count)
     t_traj = time.perf_counter() - t0
     sites = sum(2 if g[0] == "cnot" else 1 for g in gates) * 2
-    log(f"trajectories: {count} x {n} q in {t_traj:.3f} s ({int(j.sum())} jumps)")
+    log(f"trajectories (one-shot, default launch): {count} x {n} q in {t_traj:.3f} s ({int(j.sum())} jumps)")
+    # the same batch as a plan: autotuned launch configuration, one run = one hipGraph replay
+    tp = time.perf_counter()
+    plan = LT.TrajectoryPlan.from_circuit(n, gates, gamma, lam, count, tune=True)
+    t_plan = time.perf_counter() - tp
+    threads, workgroups, fused, tune_ms = plan.config()
+    plan.run(1)
+    reps = 5
+    t0 = time.perf_counter()
+    for r in range(reps):
+        zp, jp, _ = plan.run(100 + r)
+    t_graph = (time.perf_counter() - t0) / reps
+    plan.close()
+    log(f"trajectories (tuned plan, graph replay): {t_graph:.4f} s per batch ({threads} threads x {workgroups} "
+        f"workgroups, fused={fused}; plan {t_plan:.2f} s incl. tuning {tune_ms / 1e3:.2f} s)")
     batch = C.mixed_batch(1, count=mixed)
     t0 = time.perf_counter()
     zb, jb, _ = LT.run_batch(batch, seed=2)
     t_mixed = time.perf_counter() - t0
     mixed_gates = sum(len(b[1]) for b in batch)
-    log(f"mixed batch: {mixed} circuits of 10..16 q in {t_mixed:.3f} s")
+    mplan = LT.TrajectoryPlan(batch, tune=True)
+    mcfg = mplan.config()
+    mplan.run(2)
+    t0 = time.perf_counter()
+    mplan.run(3)
+    t_mixed_graph = time.perf_counter() - t0
+    mplan.close()
+    log(f"mixed batch: {mixed} circuits of 10..16 q in {t_mixed:.3f} s one-shot, {t_mixed_graph:.3f} s tuned plan")
     # ensemble check at 8 qubits (density matrix on the CPU, checker only)
     from oracle import oracle as O, trajectory as T
 
@@ -227,12 +249,18 @@ def trajectory_leg(n=14, count=4096, mixed=2048):
     return {
         "workload": f"{count} trajectories, {n}-qubit brick circuit, 8 layers ({len(gates)} gates, {sites} noise "
                     f"channels per trajectory), gamma ~ U[1e-3, 1e-2], lambda = gamma / 2, Philox4x32-10",
-        "s": round(t_traj, 4),
-        "trajectory_gates_per_s": round(count * len(gates) / t_traj, 1),
-        "trajectories_per_s": round(count / t_traj, 1),
+        "s": round(t_graph, 4),
+        "trajectory_gates_per_s": round(count * len(gates) / t_graph, 1),
+        "trajectories_per_s": round(count / t_graph, 1),
         "jumps": int(j.sum()),
-        "mixed_batch": {"circuits": mixed, "qubits": "10..16", "s": round(t_mixed, 4),
-                        "gates_per_s": round(mixed_gates / t_mixed, 1)},
+        "tuned_plan": {"threads": threads, "workgroups": workgroups, "fused": fused,
+                       "tuning_s": round(tune_ms / 1e3, 3), "plan_create_s": round(t_plan, 3),
+                       "s_per_batch_graph_replay": round(t_graph, 4)},
+        "one_shot_default_launch_s": round(t_traj, 4),
+        "mixed_batch": {"circuits": mixed, "qubits": "10..16", "s": round(t_mixed_graph, 4),
+                        "gates_per_s": round(mixed_gates / t_mixed_graph, 1), "one_shot_s": round(t_mixed, 4),
+                        "tuned": {"threads": mcfg[0], "workgroups": mcfg[1], "fused": mcfg[2],
+                                  "tuning_s": round(mcfg[3] / 1e3, 3)}},
         "ensemble_check_8q_max_sigma": round(dev, 2),
     }
 

@@ -258,6 +258,34 @@ int llsv_traj_run(const llsv_traj_item* items, int n_items, const llsv_tgate* ga
                   const double* gamma, const double* lambda, int n_rates, uint64_t seed, double* z,
                   uint32_t* jumps, uint64_t* hashes, double* state);
 
+/* A batch prepared for repeated runs (no counterpart in the reference, whose TrajectorySolver
+ * samples one trajectory per call; for callers that re-run a batch with new seeds).  Uploads
+ * items, gates and rates once and chooses the launch configuration: threads per trajectory
+ * (256, 512 or 1024), persistent workgroups per CU (1, 2 or 4) and whether the noise operator
+ * drawn after a gate is applied inside the next one-qubit gate's pass (fusion) or in a pass of
+ * its own.  tune = 1: the autotuner times every candidate on a sample of this batch (about
+ * 2048 items) and keeps the fastest, cached per batch shape (qubit range, log2 of the item
+ * count and of the gates per item) for the process, so llsv_traj_run uses it too; tune = 0:
+ * the cached choice or the default; tune = -1: the default (llsv_traj_default_config).  One run
+ * (seed upload, work-counter reset, the persistent kernel, result copies) is captured in a
+ * hipGraph. */
+typedef struct llsv_traj_plan llsv_traj_plan;
+int llsv_traj_plan_create(const llsv_traj_item* items, int n_items, const llsv_tgate* gates, int n_gates,
+                          const double* gamma, const double* lambda, int n_rates, int tune,
+                          llsv_traj_plan** out);
+
+/* Replays the plan's graph with this seed; results as llsv_traj_run.  Synchronous. */
+int llsv_traj_plan_run(llsv_traj_plan* p, uint64_t seed, double* z, uint32_t* jumps, uint64_t* hashes);
+
+/* The plan's configuration: threads per trajectory, workgroups in the persistent launch,
+ * fusion (0 / 1) and the time the autotuner took in ms (0 when the choice came from the cache). */
+int llsv_traj_plan_config(const llsv_traj_plan* p, int* threads, int* workgroups, int* fused, double* tune_ms);
+int llsv_traj_plan_destroy(llsv_traj_plan* p);
+
+/* Sets the process-wide default configuration (initially 256 threads, 4 workgroups per CU,
+ * fused): threads in {256, 512, 1024}, per_cu in {1, 2, 4}, threads * per_cu <= 2048. */
+int llsv_traj_default_config(int threads, int per_cu, int fused);
+
 /* ---- interop and measurement ------------------------------------------------ */
 
 /* Device address of the amplitudes and the state's stream (hipStream_t as void*). */

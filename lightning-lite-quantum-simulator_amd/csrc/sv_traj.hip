@@ -13,9 +13,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <numeric>
+#include <tuple>
 #include <vector>
 
 #include "../../include/llsv.h"
@@ -126,9 +131,10 @@ __device__ __forceinline__ v2d ap(const double* m, v2d a0, v2d a1) {
 
 __device__ __forceinline__ double nrm2(v2d a) { return a.x * a.x + a.y * a.y; }
 
-// Sums NV values over the 256 threads of the block; the totals land in out[] (all threads).
-template <int NV>
-__device__ inline void block_sums(double* v, double (*lds)[4], double* out) {
+// Sums NV values over the WG threads of the block; the totals land in out[] (all threads).
+template <int WG, int NV>
+__device__ inline void block_sums(double* v, double (*lds)[WG / 64], double* out) {
+    constexpr int W = WG / 64;
 #pragma unroll
     for (int k = 0; k < NV; ++k)
         for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
@@ -138,23 +144,108 @@ __device__ inline void block_sums(double* v, double (*lds)[4], double* out) {
         for (int k = 0; k < NV; ++k) lds[k][w] = v[k];
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < NV; ++k) out[k] = (lds[k][0] + lds[k][1]) + (lds[k][2] + lds[k][3]);
+    for (int k = 0; k < NV; ++k) {
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i < W; i += 2) t += lds[k][i] + lds[k][i + 1];
+        out[k] = t;
+    }
     __syncthreads();
 }
 
 __global__ void k_traj_reset(unsigned* counter) { *counter = 0u; }
 
-__global__ __launch_bounds__(256) void k_traj(const TItem* __restrict__ items, int n_items,
-                                              const int* __restrict__ order, const TG* __restrict__ gates,
-                                              const double* __restrict__ gamma, const double* __restrict__ lam,
-                                              uint64_t seed, v2d* __restrict__ ws, uint64_t slot_size,
-                                              unsigned* __restrict__ counter, double* __restrict__ zout,
-                                              unsigned* __restrict__ jout, uint64_t* __restrict__ hout) {
+// One-qubit operator m on qubit q; with PROBS, v[0] / v[1] accumulate the weight of q = 0 / 1.
+template <int WG, bool PROBS>
+__device__ inline void pass_1q(v2d* s, uint64_t N, int q, const double* m, double* v) {
+    const uint64_t lomask = (uint64_t(1) << q) - 1, st = uint64_t(1) << q;
+    for (uint64_t p = threadIdx.x; p < N / 2; p += WG) {
+        const uint64_t i0 = ((p & ~lomask) << 1) | (p & lomask);
+        const v2d a0 = s[i0], a1 = s[i0 + st];
+        const v2d b0 = ap(m, a0, a1), b1 = ap(m + 4, a0, a1);
+        s[i0] = b0;
+        s[i0 + st] = b1;
+        if (PROBS) {
+            v[0] += nrm2(b0);
+            v[1] += nrm2(b1);
+        }
+    }
+}
+
+// The pending noise operator k on qubit a, then gate m on qubit b != a, in one pass over
+// groups of 4 amplitudes; v[0] / v[1] accumulate the weight of b = 0 / 1 after both.
+template <int WG>
+__device__ inline void pass_k_then_1q(v2d* s, uint64_t N, int a, const double* k, int b, const double* m, double* v) {
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    const uint64_t ab = uint64_t(1) << a, bb = uint64_t(1) << b;
+    for (uint64_t p = threadIdx.x; p < N / 4; p += WG) {
+        const uint64_t i = insert0(insert0(p, lo), hi);
+        const v2d x00 = s[i], x10 = s[i | ab], x01 = s[i | bb], x11 = s[i | ab | bb];  // x(a bit)(b bit)
+        const v2d y00 = ap(k, x00, x10), y10 = ap(k + 4, x00, x10);
+        const v2d y01 = ap(k, x01, x11), y11 = ap(k + 4, x01, x11);
+        const v2d z00 = ap(m, y00, y01), z01 = ap(m + 4, y00, y01);
+        const v2d z10 = ap(m, y10, y11), z11 = ap(m + 4, y10, y11);
+        s[i] = z00;
+        s[i | ab] = z10;
+        s[i | bb] = z01;
+        s[i | ab | bb] = z11;
+        v[0] += nrm2(z00) + nrm2(z10);
+        v[1] += nrm2(z01) + nrm2(z11);
+    }
+}
+
+// CNOT(c, t); v[] accumulates the joint weights (c, t) = 00, 01, 10, 11 after the gate.
+template <int WG>
+__device__ inline void pass_cnot(v2d* s, uint64_t N, int c, int t, double* v) {
+    const int lo = c < t ? c : t, hi = c < t ? t : c;
+    const uint64_t cb = uint64_t(1) << c, tb = uint64_t(1) << t;
+    for (uint64_t p = threadIdx.x; p < N / 4; p += WG) {
+        const uint64_t b = insert0(insert0(p, lo), hi);
+        const v2d a00 = s[b], a01 = s[b | tb], a10 = s[b | cb], a11 = s[b | cb | tb];
+        s[b | cb] = a11;  // control set: swap target
+        s[b | cb | tb] = a10;
+        v[0] += nrm2(a00);
+        v[1] += nrm2(a01);
+        v[2] += nrm2(a11);
+        v[3] += nrm2(a10);
+    }
+}
+
+// Noise operators kc on qubit c, then kt on qubit t (they act on different qubits).
+template <int WG>
+__device__ inline void pass_k2(v2d* s, uint64_t N, int c, int t, const double* kc, const double* kt) {
+    const int lo = c < t ? c : t, hi = c < t ? t : c;
+    const uint64_t cb = uint64_t(1) << c, tb = uint64_t(1) << t;
+    for (uint64_t p = threadIdx.x; p < N / 4; p += WG) {
+        const uint64_t b = insert0(insert0(p, lo), hi);
+        v2d a00 = s[b], a01 = s[b | tb], a10 = s[b | cb], a11 = s[b | cb | tb];
+        const v2d c00 = ap(kc, a00, a10), c10 = ap(kc + 4, a00, a10);
+        const v2d c01 = ap(kc, a01, a11), c11 = ap(kc + 4, a01, a11);
+        s[b] = ap(kt, c00, c01);
+        s[b | tb] = ap(kt + 4, c00, c01);
+        s[b | cb] = ap(kt, c10, c11);
+        s[b | cb | tb] = ap(kt + 4, c10, c11);
+    }
+}
+
+// WG threads per trajectory.  FUSE: the noise operator drawn after a gate is not applied in a
+// pass of its own but inside the next gate's pass when that gate is a one-qubit gate (multiplied
+// into it when both act on the same qubit); before a CNOT, and after the last gate, it gets its
+// own pass.  The operators and the order they act in are the same either way.
+template <int WG, bool FUSE>
+__global__ __launch_bounds__(WG) void k_traj(const TItem* __restrict__ items, int n_items,
+                                             const int* __restrict__ order, const TG* __restrict__ gates,
+                                             const double* __restrict__ gamma, const double* __restrict__ lam,
+                                             const uint64_t* __restrict__ seedp, v2d* __restrict__ ws,
+                                             uint64_t slot_size, unsigned* __restrict__ counter,
+                                             double* __restrict__ zout, unsigned* __restrict__ jout,
+                                             uint64_t* __restrict__ hout) {
     __shared__ int s_item;
     __shared__ double s_k[2][8];
-    __shared__ double lds[20][4];
+    __shared__ double lds[20][WG / 64];
     v2d* s = ws + uint64_t(blockIdx.x) * slot_size;
     const int tid = threadIdx.x;
+    const uint64_t seed = *seedp;
     for (;;) {
         if (tid == 0) {
             const unsigned k = atomicAdd(counter, 1u);
@@ -167,7 +258,7 @@ __global__ __launch_bounds__(256) void k_traj(const TItem* __restrict__ items, i
         const TItem item = items[it];
         const int n = item.n;
         const uint64_t N = uint64_t(1) << n;
-        for (uint64_t i = tid; i < N; i += 256) {
+        for (uint64_t i = tid; i < N; i += WG) {
             v2d z;
             z.x = (i == 0) ? 1.0 : 0.0;
             z.y = 0.0;
@@ -176,23 +267,42 @@ __global__ __launch_bounds__(256) void k_traj(const TItem* __restrict__ items, i
         __syncthreads();
         unsigned jumps = 0;
         uint64_t hash = 0xCBF29CE484222325ull;
+        int pend = 0, pq0 = 0, pq1 = 0;  // pending noise operator: none, on pq0 (s_k[0]), or on pq0 and pq1
+        auto flush = [&]() {
+            double k0[8], k1[8];
+            for (int j = 0; j < 8; ++j) {
+                k0[j] = s_k[0][j];
+                k1[j] = s_k[1][j];
+            }
+            if (pend == 1)
+                pass_1q<WG, false>(s, N, pq0, k0, nullptr);
+            else
+                pass_k2<WG>(s, N, pq0, pq1, k0, k1);
+            pend = 0;
+            __syncthreads();
+        };
         for (int gi = 0; gi < item.n_gates; ++gi) {
             const TG g = gates[item.first_gate + gi];
             if (g.kind == 0) {
                 const int q = g.q0;
-                const uint64_t lomask = (uint64_t(1) << q) - 1, st = uint64_t(1) << q;
                 double v[2] = {0.0, 0.0};
-                for (uint64_t p = tid; p < N / 2; p += 256) {
-                    const uint64_t i0 = ((p & ~lomask) << 1) | (p & lomask);
-                    const v2d a0 = s[i0], a1 = s[i0 + st];
-                    const v2d b0 = ap(g.m, a0, a1), b1 = ap(g.m + 4, a0, a1);
-                    s[i0] = b0;
-                    s[i0 + st] = b1;
-                    v[0] += nrm2(b0);
-                    v[1] += nrm2(b1);
+                if (FUSE && pend == 1) {
+                    double k[8];
+                    for (int j = 0; j < 8; ++j) k[j] = s_k[0][j];
+                    if (pq0 == q) {
+                        double mk[8];
+                        mm2(g.m, k, mk);
+                        pass_1q<WG, true>(s, N, q, mk, v);
+                    } else {
+                        pass_k_then_1q<WG>(s, N, pq0, k, q, g.m, v);
+                    }
+                    pend = 0;
+                } else {
+                    if (pend) flush();
+                    pass_1q<WG, true>(s, N, q, g.m, v);
                 }
                 double P2[2];
-                block_sums<2>(v, lds, P2);
+                block_sums<WG, 2>(v, lds, P2);
                 if (tid == 0) {
                     double P[2][2] = {{P2[0], 0.0}, {P2[1], 0.0}};
                     double k[8];
@@ -202,32 +312,16 @@ __global__ __launch_bounds__(256) void k_traj(const TItem* __restrict__ items, i
                     for (int j = 0; j < 8; ++j) s_k[0][j] = k[j] * nr;
                 }
                 __syncthreads();
-                double k[8];
-                for (int j = 0; j < 8; ++j) k[j] = s_k[0][j];
-                for (uint64_t p = tid; p < N / 2; p += 256) {
-                    const uint64_t i0 = ((p & ~lomask) << 1) | (p & lomask);
-                    const v2d a0 = s[i0], a1 = s[i0 + st];
-                    s[i0] = ap(k, a0, a1);
-                    s[i0 + st] = ap(k + 4, a0, a1);
-                }
-                __syncthreads();
+                pend = 1;
+                pq0 = q;
+                if (!FUSE) flush();
             } else {
+                if (pend) flush();
                 const int c = g.q0, t = g.q1;
-                const int lo = c < t ? c : t, hi = c < t ? t : c;
-                const uint64_t cb = uint64_t(1) << c, tb = uint64_t(1) << t;
-                double v[4] = {0.0, 0.0, 0.0, 0.0};  // weights (c, t) = 00, 01, 10, 11
-                for (uint64_t p = tid; p < N / 4; p += 256) {
-                    const uint64_t b = insert0(insert0(p, lo), hi);
-                    const v2d a00 = s[b], a01 = s[b | tb], a10 = s[b | cb], a11 = s[b | cb | tb];
-                    s[b | cb] = a11;  // control set: swap target
-                    s[b | cb | tb] = a10;
-                    v[0] += nrm2(a00);
-                    v[1] += nrm2(a01);
-                    v[2] += nrm2(a11);
-                    v[3] += nrm2(a10);
-                }
+                double v[4] = {0.0, 0.0, 0.0, 0.0};
+                pass_cnot<WG>(s, N, c, t, v);
                 double P4[4];
-                block_sums<4>(v, lds, P4);
+                block_sums<WG, 4>(v, lds, P4);
                 if (tid == 0) {
                     double P[2][2] = {{P4[0], P4[1]}, {P4[2], P4[3]}};
                     double kc[8], kt[8];
@@ -243,34 +337,22 @@ __global__ __launch_bounds__(256) void k_traj(const TItem* __restrict__ items, i
                     }
                 }
                 __syncthreads();
-                double kc[8], kt[8];
-                for (int j = 0; j < 8; ++j) {
-                    kc[j] = s_k[0][j];
-                    kt[j] = s_k[1][j];
-                }
-                for (uint64_t p = tid; p < N / 4; p += 256) {
-                    const uint64_t b = insert0(insert0(p, lo), hi);
-                    v2d a00 = s[b], a01 = s[b | tb], a10 = s[b | cb], a11 = s[b | cb | tb];
-                    // control qubit first, then target (the operators act on different qubits)
-                    const v2d c00 = ap(kc, a00, a10), c10 = ap(kc + 4, a00, a10);
-                    const v2d c01 = ap(kc, a01, a11), c11 = ap(kc + 4, a01, a11);
-                    s[b] = ap(kt, c00, c01);
-                    s[b | tb] = ap(kt + 4, c00, c01);
-                    s[b | cb] = ap(kt, c10, c11);
-                    s[b | cb | tb] = ap(kt + 4, c10, c11);
-                }
-                __syncthreads();
+                pend = 2;
+                pq0 = c;
+                pq1 = t;
+                if (!FUSE) flush();
             }
         }
+        if (pend) flush();
         double zq[20];
         for (int q = 0; q < 20; ++q) zq[q] = 0.0;
-        for (uint64_t i = tid; i < N; i += 256) {
+        for (uint64_t i = tid; i < N; i += WG) {
             const double p = nrm2(s[i]);
             for (int q = 0; q < 20; ++q)
                 if (q < n) zq[q] += ((i >> q) & 1) ? -p : p;
         }
         double Z[20];
-        block_sums<20>(zq, lds, Z);
+        block_sums<WG, 20>(zq, lds, Z);
         if (tid == 0) {
             for (int q = 0; q < n; ++q) zout[size_t(it) * LLSV_TRAJ_ZSTRIDE + q] = Z[q];
             jout[it] = jumps;
@@ -280,106 +362,395 @@ __global__ __launch_bounds__(256) void k_traj(const TItem* __restrict__ items, i
     }
 }
 
+// ---------------------------------------------------------------- host: buffers, launch, tuning
+struct TrajConfig {
+    int wg = 256;       // threads per trajectory (workgroup size)
+    int per_cu = 4;     // persistent workgroups per CU
+    int fuse = 1;       // noise operator applied inside the next one-qubit gate's pass
+};
+
+int device_cus() {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    return cus;
+}
+
+// Workgroups of a launch: at most n_items, and at most what a 4 GiB workspace holds.
+int traj_blocks(const TrajConfig& c, int n_items, int maxn, int cus) {
+    const uint64_t slot_bytes = uint64_t(16) << maxn;
+    const uint64_t cap = std::max<uint64_t>(1, (uint64_t(4) << 30) / slot_bytes);
+    return int(std::min<uint64_t>({uint64_t(n_items), uint64_t(c.per_cu) * uint64_t(cus), cap}));
+}
+
+hipError_t launch_traj(const TrajConfig& c, int blocks, hipStream_t st, const TItem* items, int n_items,
+                       const int* order, const TG* gates, const double* gamma, const double* lam, const uint64_t* seed,
+                       v2d* ws, uint64_t slot, unsigned* counter, double* z, unsigned* j, uint64_t* h) {
+    hipLaunchKernelGGL(k_traj_reset, dim3(1), dim3(1), 0, st, counter);
+#define LLSV_TRAJ_LAUNCH(WG, F)                                                                                   \
+    hipLaunchKernelGGL((k_traj<WG, F>), dim3(blocks), dim3(WG), 0, st, items, n_items, order, gates, gamma, lam, \
+                       seed, ws, slot, counter, z, j, h)
+    if (c.wg == 1024) {
+        if (c.fuse) LLSV_TRAJ_LAUNCH(1024, true); else LLSV_TRAJ_LAUNCH(1024, false);
+    } else if (c.wg == 512) {
+        if (c.fuse) LLSV_TRAJ_LAUNCH(512, true); else LLSV_TRAJ_LAUNCH(512, false);
+    } else {
+        if (c.fuse) LLSV_TRAJ_LAUNCH(256, true); else LLSV_TRAJ_LAUNCH(256, false);
+    }
+#undef LLSV_TRAJ_LAUNCH
+    return hipGetLastError();
+}
+
+// Shape of a batch for the tuning cache: qubit range, log2 of the item count and of the mean
+// gate count per item.
+struct TrajShape {
+    int maxn, minn, lg_items, lg_gates;
+    bool operator<(const TrajShape& o) const {
+        return std::tie(maxn, minn, lg_items, lg_gates) < std::tie(o.maxn, o.minn, o.lg_items, o.lg_gates);
+    }
+};
+
+std::mutex g_tune_mu;
+TrajConfig g_default;  // used when no tuned choice is cached (guarded by g_tune_mu)
+std::map<TrajShape, TrajConfig>& tune_cache() {
+    static auto* m = new std::map<TrajShape, TrajConfig>();
+    return *m;
+}
+
+int ilog2(uint64_t x) { return x ? 63 - __builtin_clzll(x) : 0; }
+
+// Validates a batch; fills the largest-first order, the qubit range and the shape.
+int check_batch(const llsv_traj_item* items, int n_items, const llsv_tgate* gates, int n_gates, const double* gamma,
+                const double* lambda, int n_rates, const char* who, std::vector<int>* order, TrajShape* shape) {
+    if (!items || n_items < 1 || !gates || n_gates < 1 || !gamma || !lambda || n_rates < 1)
+        return fail(LLSV_ERR_ARG, "%s: empty or null argument", who);
+    int maxn = 0, minn = 99;
+    uint64_t gsum = 0;
+    for (int i = 0; i < n_items; ++i) {
+        const llsv_traj_item& it = items[i];
+        if (it.n_qubits < 1 || it.n_qubits > 20) return fail(LLSV_ERR_ARG, "%s: item %d: n_qubits must be in [1, 20]", who, i);
+        if (it.first_gate < 0 || it.n_gates < 0 || it.first_gate + it.n_gates > n_gates)
+            return fail(LLSV_ERR_ARG, "%s: item %d: gate range outside the gate list", who, i);
+        if (it.first_rate < 0 || it.first_rate + it.n_qubits > n_rates)
+            return fail(LLSV_ERR_ARG, "%s: item %d: rate range outside the rate lists", who, i);
+        for (int k = 0; k < it.n_gates; ++k) {
+            const llsv_tgate& g = gates[it.first_gate + k];
+            const bool ok = (g.kind == 0 && g.q0 >= 0 && g.q0 < it.n_qubits) ||
+                            (g.kind == 1 && g.q0 >= 0 && g.q0 < it.n_qubits && g.q1 >= 0 && g.q1 < it.n_qubits && g.q0 != g.q1);
+            if (!ok) return fail(LLSV_ERR_ARG, "%s: item %d gate %d: bad kind or qubits", who, i, k);
+        }
+        for (int q = 0; q < it.n_qubits; ++q) {
+            const double g = gamma[it.first_rate + q], l = lambda[it.first_rate + q];
+            if (!(g >= 0.0 && g <= 1.0 && l >= 0.0 && l <= 1.0))
+                return fail(LLSV_ERR_ARG, "%s: item %d: rates must be in [0, 1]", who, i);
+        }
+        maxn = std::max(maxn, it.n_qubits);
+        minn = std::min(minn, it.n_qubits);
+        gsum += uint64_t(it.n_gates);
+    }
+    order->resize(n_items);
+    std::iota(order->begin(), order->end(), 0);
+    std::stable_sort(order->begin(), order->end(), [&](int a, int b) { return items[a].n_qubits > items[b].n_qubits; });
+    *shape = TrajShape{maxn, minn, ilog2(uint64_t(n_items)), ilog2(gsum / uint64_t(n_items) + 1)};
+    return LLSV_OK;
+}
+
+// Device copy of one batch: inputs, results, work counter, seed and the per-workgroup state slots.
+struct TrajDev {
+    unsigned char* buf = nullptr;
+    TItem* items = nullptr;
+    int* order = nullptr;
+    int* sample = nullptr;  // tuning sample order
+    TG* gates = nullptr;
+    double *gamma = nullptr, *lam = nullptr, *z = nullptr;
+    unsigned *j = nullptr, *counter = nullptr;
+    uint64_t *h = nullptr, *seed = nullptr;
+    v2d* ws = nullptr;
+    uint64_t slot = 0;
+    int max_blocks = 0;
+    ~TrajDev() {
+        if (buf) (void)hipFree(buf);
+    }
+};
+
+int traj_upload(TrajDev* d, const llsv_traj_item* items, int n_items, const std::vector<int>& order,
+                const llsv_tgate* gates, int n_gates, const double* gamma, const double* lambda, int n_rates,
+                int maxn, int max_blocks, hipStream_t st, const char* who) {
+    const size_t b_items = sizeof(llsv_traj_item) * n_items, b_order = sizeof(int) * n_items;
+    const size_t b_gates = sizeof(llsv_tgate) * n_gates, b_rates = sizeof(double) * n_rates;
+    const size_t b_z = sizeof(double) * LLSV_TRAJ_ZSTRIDE * n_items, b_j = sizeof(unsigned) * n_items;
+    const size_t b_h = sizeof(uint64_t) * n_items;
+    d->slot = uint64_t(1) << maxn;
+    d->max_blocks = max_blocks;
+    const size_t b_ws = d->slot * 16 * size_t(max_blocks);
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t total = al(b_items) + 2 * al(b_order) + al(b_gates) + 2 * al(b_rates) + al(b_z) + al(b_j) + al(b_h) +
+                         512 + al(b_ws);
+    if (hipMalloc(reinterpret_cast<void**>(&d->buf), total) != hipSuccess) {
+        (void)hipGetLastError();
+        d->buf = nullptr;
+        return fail(LLSV_ERR_NOMEM, "%s: cannot allocate %zu bytes", who, total);
+    }
+    size_t off = 0;
+    auto take = [&](size_t b) { unsigned char* p = d->buf + off; off += al(b); return p; };
+    d->items = reinterpret_cast<TItem*>(take(b_items));
+    d->order = reinterpret_cast<int*>(take(b_order));
+    d->sample = reinterpret_cast<int*>(take(b_order));
+    d->gates = reinterpret_cast<TG*>(take(b_gates));
+    d->gamma = reinterpret_cast<double*>(take(b_rates));
+    d->lam = reinterpret_cast<double*>(take(b_rates));
+    d->z = reinterpret_cast<double*>(take(b_z));
+    d->j = reinterpret_cast<unsigned*>(take(b_j));
+    d->h = reinterpret_cast<uint64_t*>(take(b_h));
+    d->counter = reinterpret_cast<unsigned*>(take(256));
+    d->seed = reinterpret_cast<uint64_t*>(take(256));
+    d->ws = reinterpret_cast<v2d*>(take(b_ws));
+    hipError_t e = hipMemcpyAsync(d->items, items, b_items, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d->order, order.data(), b_order, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d->gates, gates, b_gates, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d->gamma, gamma, b_rates, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(d->lam, lambda, b_rates, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail(LLSV_ERR_DEVICE, "%s: upload: %s", who, hipGetErrorString(e));
+    return LLSV_OK;
+}
+
+// The autotuner: times every candidate configuration (threads per trajectory 256 / 512 / 1024,
+// 1 / 2 / 4 persistent workgroups per CU, noise operator fused or in its own pass) on a sample of
+// the batch -- every k-th item, about 2048 in all, largest first -- and returns the fastest.
+int traj_tune(TrajDev& d, const llsv_traj_item* items, int n_items, const std::vector<int>& order, int maxn,
+              hipStream_t st, TrajConfig* best, double* tune_ms) {
+    const int cus = device_cus();
+    std::vector<int> sample;
+    const int step = std::max(1, n_items / 2048);
+    for (int i = 0; i < n_items; i += step) sample.push_back(order[i]);  // keeps the largest-first order
+    hipError_t e = hipMemcpyAsync(d.sample, sample.data(), sizeof(int) * sample.size(), hipMemcpyHostToDevice, st);
+    const uint64_t seed_probe = 0x5EEDull;
+    if (e == hipSuccess) e = hipMemcpyAsync(d.seed, &seed_probe, sizeof seed_probe, hipMemcpyHostToDevice, st);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    auto t_start = std::chrono::steady_clock::now();
+    float best_ms = 1e30f;
+    for (int wg : {256, 512, 1024})
+        for (int per_cu : {1, 2, 4})
+            for (int fuse : {0, 1}) {
+                if (e != hipSuccess) break;
+                if (wg * per_cu > 2048) continue;  // more threads than one CU holds at once
+                const TrajConfig c{wg, per_cu, fuse};
+                const int blocks = std::min(traj_blocks(c, int(sample.size()), maxn, cus), d.max_blocks);
+                float ms = 0.0f;
+                for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {  // first run warms up
+                    e = hipEventRecord(e0, st);
+                    if (e == hipSuccess)
+                        e = launch_traj(c, blocks, st, d.items, int(sample.size()), d.sample, d.gates, d.gamma, d.lam,
+                                        d.seed, d.ws, d.slot, d.counter, d.z, d.j, d.h);
+                    if (e == hipSuccess) e = hipEventRecord(e1, st);
+                    if (e == hipSuccess) e = hipEventSynchronize(e1);
+                    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                }
+                if (e == hipSuccess && ms < best_ms) {
+                    best_ms = ms;
+                    *best = c;
+                }
+            }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    *tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    if (e != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_traj_plan_create: tuning: %s", hipGetErrorString(e));
+    (void)items;
+    return LLSV_OK;
+}
+
 }  // namespace
 }  // namespace llsv
+
+// A prepared batch: device data, the chosen configuration and the captured run.
+struct llsv_traj_plan {
+    llsv::TrajDev dev;
+    llsv::TrajConfig cfg;
+    int n_items = 0, blocks = 0, maxn = 0;
+    double tune_ms = 0.0;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t* h_seed = nullptr;  // pinned
+    double* h_z = nullptr;       // pinned results
+    unsigned* h_j = nullptr;
+    uint64_t* h_h = nullptr;
+    ~llsv_traj_plan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (h_seed) (void)hipHostFree(h_seed);
+        if (h_z) (void)hipHostFree(h_z);
+        if (h_j) (void)hipHostFree(h_j);
+        if (h_h) (void)hipHostFree(h_h);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+
 
 extern "C" int llsv_traj_run(const llsv_traj_item* items, int n_items, const llsv_tgate* gates, int n_gates,
                              const double* gamma, const double* lambda, int n_rates, uint64_t seed, double* z,
                              uint32_t* jumps, uint64_t* hashes, double* state) {
     using namespace llsv;
-    if (!items || n_items < 1 || !gates || n_gates < 1 || !gamma || !lambda || n_rates < 1 || !z)
-        return fail(LLSV_ERR_ARG, "llsv_traj_run: empty or null argument");
-    int maxn = 0;
-    for (int i = 0; i < n_items; ++i) {
-        const llsv_traj_item& it = items[i];
-        if (it.n_qubits < 1 || it.n_qubits > 20) return fail(LLSV_ERR_ARG, "llsv_traj_run: item %d: n_qubits must be in [1, 20]", i);
-        if (it.first_gate < 0 || it.n_gates < 0 || it.first_gate + it.n_gates > n_gates)
-            return fail(LLSV_ERR_ARG, "llsv_traj_run: item %d: gate range outside the gate list", i);
-        if (it.first_rate < 0 || it.first_rate + it.n_qubits > n_rates)
-            return fail(LLSV_ERR_ARG, "llsv_traj_run: item %d: rate range outside the rate lists", i);
-        for (int k = 0; k < it.n_gates; ++k) {
-            const llsv_tgate& g = gates[it.first_gate + k];
-            const bool ok = (g.kind == 0 && g.q0 >= 0 && g.q0 < it.n_qubits) ||
-                            (g.kind == 1 && g.q0 >= 0 && g.q0 < it.n_qubits && g.q1 >= 0 && g.q1 < it.n_qubits && g.q0 != g.q1);
-            if (!ok) return fail(LLSV_ERR_ARG, "llsv_traj_run: item %d gate %d: bad kind or qubits", i, k);
-        }
-        for (int q = 0; q < it.n_qubits; ++q) {
-            const double g = gamma[it.first_rate + q], l = lambda[it.first_rate + q];
-            if (!(g >= 0.0 && g <= 1.0 && l >= 0.0 && l <= 1.0))
-                return fail(LLSV_ERR_ARG, "llsv_traj_run: item %d: rates must be in [0, 1]", i);
-        }
-        maxn = std::max(maxn, it.n_qubits);
+    if (!z) return fail(LLSV_ERR_ARG, "llsv_traj_run: z is null");
+    std::vector<int> order;
+    TrajShape shape{};
+    if (int rc = check_batch(items, n_items, gates, n_gates, gamma, lambda, n_rates, "llsv_traj_run", &order, &shape))
+        return rc;
+    TrajConfig cfg;  // a configuration the tuner chose for this shape earlier in the process, else the default
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        auto it = tune_cache().find(shape);
+        cfg = it != tune_cache().end() ? it->second : g_default;
     }
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const int blocks = std::min(n_items, 4 * cus);
-    const uint64_t slot = uint64_t(1) << maxn;
-    // largest items first
-    std::vector<int> order(n_items);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return items[a].n_qubits > items[b].n_qubits; });
-
-    const size_t b_items = sizeof(llsv_traj_item) * n_items, b_order = sizeof(int) * n_items;
-    const size_t b_gates = sizeof(llsv_tgate) * n_gates, b_rates = sizeof(double) * n_rates;
-    const size_t b_z = sizeof(double) * LLSV_TRAJ_ZSTRIDE * n_items, b_j = sizeof(unsigned) * n_items;
-    const size_t b_h = sizeof(uint64_t) * n_items, b_ws = slot * 16 * size_t(blocks);
-    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
-    const size_t total = al(b_items) + al(b_order) + al(b_gates) + 2 * al(b_rates) + al(b_z) + al(b_j) + al(b_h) + 256 + al(b_ws);
-    unsigned char* buf = nullptr;
-    if (hipMalloc(reinterpret_cast<void**>(&buf), total) != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(LLSV_ERR_NOMEM, "llsv_traj_run: cannot allocate %zu bytes", total);
-    }
-    size_t off = 0;
-    auto take = [&](size_t b) { unsigned char* p = buf + off; off += al(b); return p; };
-    TItem* d_items = reinterpret_cast<TItem*>(take(b_items));
-    int* d_order = reinterpret_cast<int*>(take(b_order));
-    TG* d_gates = reinterpret_cast<TG*>(take(b_gates));
-    double* d_gamma = reinterpret_cast<double*>(take(b_rates));
-    double* d_lam = reinterpret_cast<double*>(take(b_rates));
-    double* d_z = reinterpret_cast<double*>(take(b_z));
-    unsigned* d_j = reinterpret_cast<unsigned*>(take(b_j));
-    uint64_t* d_h = reinterpret_cast<uint64_t*>(take(b_h));
-    unsigned* d_counter = reinterpret_cast<unsigned*>(take(256));
-    v2d* d_ws = reinterpret_cast<v2d*>(take(b_ws));
+    const int cus = device_cus();
+    const int blocks = traj_blocks(cfg, n_items, shape.maxn, cus);
     hipStream_t st = nullptr;
-    hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_items, items, b_items, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_order, order.data(), b_order, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_gates, gates, b_gates, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_gamma, gamma, b_rates, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(d_lam, lambda, b_rates, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_traj_reset, dim3(1), dim3(1), 0, st, d_counter);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_traj, dim3(blocks), dim3(256), 0, st, d_items, n_items, d_order, d_gates, d_gamma, d_lam,
-                           seed, d_ws, slot, d_counter, d_z, d_j, d_h);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipMemcpyAsync(z, d_z, b_z, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess && jumps) e = hipMemcpyAsync(jumps, d_j, b_j, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess && hashes) e = hipMemcpyAsync(hashes, d_h, b_h, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess && state) {
-        // item 0's final state is in the slot of whichever workgroup ran it last: rerun it alone.
-        e = hipStreamSynchronize(st);
-        if (e == hipSuccess) {
-            const int zero = 0;
-            e = hipMemcpyAsync(d_order, &zero, sizeof(int), hipMemcpyHostToDevice, st);
-            if (e == hipSuccess) {
-                hipLaunchKernelGGL(k_traj_reset, dim3(1), dim3(1), 0, st, d_counter);
-                hipLaunchKernelGGL(k_traj, dim3(1), dim3(256), 0, st, d_items, 1, d_order, d_gates, d_gamma, d_lam,
-                                   seed, d_ws, slot, d_counter, d_z, d_j, d_h);
-                e = hipGetLastError();
-            }
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+        return fail(LLSV_ERR_DEVICE, "llsv_traj_run: stream creation failed");
+    int rc = LLSV_OK;
+    {
+        TrajDev d;
+        rc = traj_upload(&d, items, n_items, order, gates, n_gates, gamma, lambda, n_rates, shape.maxn, blocks, st,
+                         "llsv_traj_run");
+        hipError_t e = hipSuccess;
+        if (rc == LLSV_OK) {
+            e = hipMemcpyAsync(d.seed, &seed, sizeof seed, hipMemcpyHostToDevice, st);
             if (e == hipSuccess)
-                e = hipMemcpyAsync(state, d_ws, sizeof(v2d) << items[0].n_qubits, hipMemcpyDeviceToHost, st);
+                e = launch_traj(cfg, blocks, st, d.items, n_items, d.order, d.gates, d.gamma, d.lam, d.seed, d.ws,
+                                d.slot, d.counter, d.z, d.j, d.h);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(z, d.z, sizeof(double) * LLSV_TRAJ_ZSTRIDE * n_items, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess && jumps) e = hipMemcpyAsync(jumps, d.j, sizeof(unsigned) * n_items, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess && hashes) e = hipMemcpyAsync(hashes, d.h, sizeof(uint64_t) * n_items, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess && state) {
+                // item 0's final state is in the slot of whichever workgroup ran it last: rerun it alone.
+                e = hipStreamSynchronize(st);
+                const int zero = 0;
+                if (e == hipSuccess) e = hipMemcpyAsync(d.sample, &zero, sizeof(int), hipMemcpyHostToDevice, st);
+                if (e == hipSuccess)
+                    e = launch_traj(cfg, 1, st, d.items, 1, d.sample, d.gates, d.gamma, d.lam, d.seed, d.ws, d.slot,
+                                    d.counter, d.z, d.j, d.h);
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(state, d.ws, sizeof(v2d) << items[0].n_qubits, hipMemcpyDeviceToHost, st);
+            }
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) rc = fail(LLSV_ERR_DEVICE, "llsv_traj_run: %s", hipGetErrorString(e));
         }
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (st) (void)hipStreamDestroy(st);
-    (void)hipFree(buf);
-    if (e != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_traj_run: %s", hipGetErrorString(e));
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
+extern "C" int llsv_traj_plan_create(const llsv_traj_item* items, int n_items, const llsv_tgate* gates, int n_gates,
+                                     const double* gamma, const double* lambda, int n_rates, int tune,
+                                     llsv_traj_plan** out) {
+    using namespace llsv;
+    if (!out) return fail(LLSV_ERR_ARG, "llsv_traj_plan_create: out is null");
+    *out = nullptr;
+    std::vector<int> order;
+    TrajShape shape{};
+    if (int rc = check_batch(items, n_items, gates, n_gates, gamma, lambda, n_rates, "llsv_traj_plan_create", &order,
+                             &shape))
+        return rc;
+    std::unique_ptr<llsv_traj_plan> p(new llsv_traj_plan());
+    p->n_items = n_items;
+    p->maxn = shape.maxn;
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(LLSV_ERR_DEVICE, "llsv_traj_plan_create: stream creation failed");
+    const int cus = device_cus();
+    // workspace for the largest candidate grid
+    const int max_blocks = traj_blocks(TrajConfig{256, 4, 1}, n_items, shape.maxn, cus);
+    if (int rc = traj_upload(&p->dev, items, n_items, order, gates, n_gates, gamma, lambda, n_rates, shape.maxn,
+                             max_blocks, p->stream, "llsv_traj_plan_create"))
+        return rc;
+    bool cached = false;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        auto it = tune_cache().find(shape);
+        p->cfg = g_default;
+        if (tune >= 0 && it != tune_cache().end()) {
+            p->cfg = it->second;
+            cached = true;
+        }
+    }
+    if (tune == 1 && !cached) {
+        if (int rc = traj_tune(p->dev, items, n_items, order, shape.maxn, p->stream, &p->cfg, &p->tune_ms)) return rc;
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        tune_cache()[shape] = p->cfg;
+    }
+    p->blocks = std::min(traj_blocks(p->cfg, n_items, shape.maxn, cus), max_blocks);
+    if (hipHostMalloc(reinterpret_cast<void**>(&p->h_seed), 256) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&p->h_z), sizeof(double) * LLSV_TRAJ_ZSTRIDE * n_items) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&p->h_j), sizeof(unsigned) * n_items) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&p->h_h), sizeof(uint64_t) * n_items) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(LLSV_ERR_NOMEM, "llsv_traj_plan_create: cannot allocate pinned result buffers");
+    }
+    // Capture one run: seed upload, counter reset (a kernel, not a memset), the persistent kernel,
+    // result copies into the pinned buffers.
+    TrajDev& d = p->dev;
+    hipError_t e = hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        hipError_t ec = hipMemcpyAsync(d.seed, p->h_seed, sizeof(uint64_t), hipMemcpyHostToDevice, p->stream);
+        if (ec == hipSuccess)
+            ec = launch_traj(p->cfg, p->blocks, p->stream, d.items, n_items, d.order, d.gates, d.gamma, d.lam, d.seed,
+                             d.ws, d.slot, d.counter, d.z, d.j, d.h);
+        if (ec == hipSuccess)
+            ec = hipMemcpyAsync(p->h_z, d.z, sizeof(double) * LLSV_TRAJ_ZSTRIDE * n_items, hipMemcpyDeviceToHost, p->stream);
+        if (ec == hipSuccess) ec = hipMemcpyAsync(p->h_j, d.j, sizeof(unsigned) * n_items, hipMemcpyDeviceToHost, p->stream);
+        if (ec == hipSuccess) ec = hipMemcpyAsync(p->h_h, d.h, sizeof(uint64_t) * n_items, hipMemcpyDeviceToHost, p->stream);
+        e = hipStreamEndCapture(p->stream, &p->graph);
+        if (ec != hipSuccess) e = ec;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(LLSV_ERR_DEVICE, "llsv_traj_plan_create: graph capture: %s", hipGetErrorString(e));
+    }
+    *out = p.release();
+    return LLSV_OK;
+}
+
+extern "C" int llsv_traj_default_config(int threads, int per_cu, int fused) {
+    using namespace llsv;
+    if ((threads != 256 && threads != 512 && threads != 1024) || (per_cu != 1 && per_cu != 2 && per_cu != 4) ||
+        (fused != 0 && fused != 1) || threads * per_cu > 2048)
+        return fail(LLSV_ERR_ARG, "llsv_traj_default_config: threads in {256, 512, 1024}, per_cu in {1, 2, 4}, "
+                                  "threads * per_cu <= 2048, fused in {0, 1}");
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_default = TrajConfig{threads, per_cu, fused};
+    return LLSV_OK;
+}
+
+extern "C" int llsv_traj_plan_run(llsv_traj_plan* p, uint64_t seed, double* z, uint32_t* jumps, uint64_t* hashes) {
+    using namespace llsv;
+    if (!p) return fail(LLSV_ERR_HANDLE, "llsv_traj_plan_run: null plan");
+    if (!z) return fail(LLSV_ERR_ARG, "llsv_traj_plan_run: z is null");
+    *p->h_seed = seed;
+    hipError_t e = hipGraphLaunch(p->exec, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_traj_plan_run: %s", hipGetErrorString(e));
+    const size_t k = size_t(p->n_items);
+    std::memcpy(z, p->h_z, sizeof(double) * LLSV_TRAJ_ZSTRIDE * k);
+    if (jumps) std::memcpy(jumps, p->h_j, sizeof(unsigned) * k);
+    if (hashes) std::memcpy(hashes, p->h_h, sizeof(uint64_t) * k);
+    return LLSV_OK;
+}
+
+extern "C" int llsv_traj_plan_config(const llsv_traj_plan* p, int* threads, int* workgroups, int* fused, double* tune_ms) {
+    if (!p) return llsv::fail(LLSV_ERR_HANDLE, "llsv_traj_plan_config: null plan");
+    if (threads) *threads = p->cfg.wg;
+    if (workgroups) *workgroups = p->blocks;
+    if (fused) *fused = p->cfg.fuse;
+    if (tune_ms) *tune_ms = p->tune_ms;
+    return LLSV_OK;
+}
+
+extern "C" int llsv_traj_plan_destroy(llsv_traj_plan* p) {
+    delete p;
     return LLSV_OK;
 }

@@ -54,6 +54,11 @@ SIGNATURES = {
     "llsv_pc_energy": (_I, [_I, _I, _P, _I, _I, _P, _P, _I, _P, _P, _U64, _P]),
     "llsv_pc_adjoint": (_I, [_I, _I, _P, _I, _I, _P, _P, _I, _P, _P, _P, _U64, _P]),
     "llsv_traj_run": (_I, [_P, _I, _P, _I, _P, _P, _I, _U64, _P, _P, _P, _P]),
+    "llsv_traj_plan_create": (_I, [_P, _I, _P, _I, _P, _P, _I, _I, ctypes.POINTER(_P)]),
+    "llsv_traj_plan_run": (_I, [_P, _U64, _P, _P, _P]),
+    "llsv_traj_plan_config": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_D)]),
+    "llsv_traj_plan_destroy": (_I, [_P]),
+    "llsv_traj_default_config": (_I, [_I, _I, _I]),
     "llsv_mark": (_I, [_P, _I]),
     "llsv_mark_elapsed": (_I, [_P, _I, _I, ctypes.POINTER(ctypes.c_float)]),
     "llsv_device_ptr": (_P, [_P]),

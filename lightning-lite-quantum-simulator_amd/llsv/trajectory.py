@@ -50,8 +50,7 @@ def pack_gates(gates, out=None, start=0):
     return arr
 
 
-def run_batch(circuits, seed: int):
-    """circuits: list of (n, gates, gamma, lam, traj_id).  Returns (z (len, 32), jumps, hashes)."""
+def _pack_batch(circuits):
     total_gates = sum(len(c[1]) for c in circuits)
     gates = (TGate * total_gates)()
     items = (TItem * len(circuits))()
@@ -64,7 +63,78 @@ def run_batch(circuits, seed: int):
         gpos += len(gl)
         gam += list(gm)
         lam += list(lm)
-    return _run(items, gates, np.asarray(gam, np.float64), np.asarray(lam, np.float64), seed)
+    return items, gates, np.asarray(gam, np.float64), np.asarray(lam, np.float64)
+
+
+def set_default_config(threads: int, per_cu: int, fused: bool):
+    """Process-wide default launch configuration (llsv_traj_default_config)."""
+    _native.check(_native.lib().llsv_traj_default_config(threads, per_cu, int(bool(fused))))
+
+
+class TrajectoryPlan:
+    """A batch prepared for repeated runs (llsv_traj_plan_*): uploaded once, launch configuration
+    chosen by the autotuner (tune=1 / True), the cache or the default (0), or the default only (-1);
+    one run captured in a hipGraph.
+
+    circuits: list of (n, gates, gamma, lam, traj_id), as for run_batch; or use from_circuit."""
+
+    def __init__(self, circuits, tune=True):
+        self._items, self._gates, self._gam, self._lam = _pack_batch(circuits)
+        self._create(tune)
+
+    @classmethod
+    def from_circuit(cls, n, gates, gamma, lam, n_traj: int, first_traj: int = 0, tune=True):
+        self = cls.__new__(cls)
+        self._gates = pack_gates(gates)
+        self._items = (TItem * n_traj)()
+        for i in range(n_traj):
+            it = self._items[i]
+            it.n_qubits, it.first_gate, it.n_gates, it.first_rate, it.traj = n, 0, len(gates), 0, first_traj + i
+        self._gam = np.asarray(gamma, np.float64)
+        self._lam = np.asarray(lam, np.float64)
+        self._create(tune)
+        return self
+
+    def _create(self, tune):
+        L = _native.lib()
+        P = ctypes.c_void_p
+        self._lib = L
+        self._h = P()
+        _native.check(L.llsv_traj_plan_create(ctypes.cast(self._items, P), len(self._items), ctypes.cast(self._gates, P),
+                                              len(self._gates), self._gam.ctypes.data_as(P), self._lam.ctypes.data_as(P),
+                                              int(self._gam.size), int(tune), ctypes.byref(self._h)))
+
+    def config(self):
+        """(threads per trajectory, workgroups, fused, tuning ms)."""
+        t, w, f, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _native.check(self._lib.llsv_traj_plan_config(self._h, ctypes.byref(t), ctypes.byref(w), ctypes.byref(f),
+                                                      ctypes.byref(ms)))
+        return t.value, w.value, bool(f.value), ms.value
+
+    def run(self, seed: int):
+        """(z (items, 32), jumps, hashes) for this seed."""
+        k = len(self._items)
+        z = np.zeros((k, ZSTRIDE))
+        j = np.zeros(k, dtype=np.uint32)
+        h = np.zeros(k, dtype=np.uint64)
+        P = ctypes.c_void_p
+        _native.check(self._lib.llsv_traj_plan_run(self._h, seed, z.ctypes.data_as(P), j.ctypes.data_as(P),
+                                                   h.ctypes.data_as(P)))
+        return z, j, h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.llsv_traj_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def run_batch(circuits, seed: int):
+    """circuits: list of (n, gates, gamma, lam, traj_id).  Returns (z (len, 32), jumps, hashes)."""
+    items, gates, gam, lam = _pack_batch(circuits)
+    return _run(items, gates, gam, lam, seed)
 
 
 def run_trajectories(n, gates, gamma, lam, seed: int, n_traj: int, first_traj: int = 0, want_state=False):

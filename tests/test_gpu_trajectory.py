@@ -80,3 +80,75 @@ def test_bad_arguments():
         LT.run_trajectories(3, [("h", 3)], [0.1] * 3, [0.1] * 3, seed=1, n_traj=2)
     with pytest.raises(ValueError):
         LT.run_trajectories(3, [("h", 0)], [1.5] * 3, [0.1] * 3, seed=1, n_traj=2)
+
+
+# ------------------------------------------------------------- autotuned plans replayed as hipGraphs
+CONFIGS = [(t, p, f) for t in (256, 512, 1024) for p in (1, 2, 4) for f in (0, 1) if t * p <= 2048]
+
+
+@pytest.mark.parametrize("threads,per_cu,fused", CONFIGS)
+def test_every_launch_configuration_matches_cpu(threads, per_cu, fused):
+    # each configuration the autotuner may pick: same decisions as the CPU oracle, <Z> within 1e-12
+    from llsv import trajectory as LT
+
+    n, count = 9, 24
+    gates, _, _ = C.noisy_brick(21, n, 5)
+    rng = np.random.default_rng(77)
+    gamma = rng.uniform(0.02, 0.2, n)
+    lam = gamma / 2
+    say(f"plan config {threads}x{per_cu} fused={fused}")
+    LT.set_default_config(threads, per_cu, fused)
+    try:
+        plan = LT.TrajectoryPlan.from_circuit(n, gates, gamma, lam, count, tune=-1)
+        t, w, f, _ = plan.config()
+        assert (t, f) == (threads, bool(fused)) and 1 <= w <= count
+        z, j, h = plan.run(5)
+        plan.close()
+    finally:
+        LT.set_default_config(256, 4, True)
+    packed = T.pack(as_u(gates))
+    assert j.sum() > 0
+    for i in range(count):
+        zc, jc, hc, _ = T.run(n, as_u(gates), gamma, lam, 5, i, packed)
+        assert j[i] == jc and h[i] == hc, f"trajectory {i}: decisions differ"
+        assert np.max(np.abs(z[i, :n] - zc)) < 1e-12
+
+
+def test_tuned_plan_replays_graph():
+    # autotuned plan: replaying the graph gives the one-shot results for each seed, and the same
+    # seed twice gives identical results
+    from llsv import trajectory as LT
+
+    n, count = 12, 512
+    gates, _, _ = C.noisy_brick(4, n, 6)
+    gamma = np.random.default_rng(12).uniform(1e-3, 2e-2, n)
+    lam = gamma / 2
+    say("tuned plan n=12 x512")
+    plan = LT.TrajectoryPlan.from_circuit(n, gates, gamma, lam, count, tune=True)
+    threads, workgroups, fused, tune_ms = plan.config()
+    assert threads in (256, 512, 1024) and 1 <= workgroups <= count and tune_ms > 0
+    for seed in (1, 2):
+        z, j, h = plan.run(seed)
+        z2, j2, h2 = plan.run(seed)
+        assert np.array_equal(z, z2) and np.array_equal(j, j2) and np.array_equal(h, h2)
+        zr, jr, hr = LT.run_trajectories(n, gates, gamma, lam, seed=seed, n_traj=count)
+        assert np.array_equal(j, jr) and np.array_equal(h, hr)
+        assert np.max(np.abs(z - zr)) < 1e-12
+    # a second plan of the same shape reuses the tuned choice without timing again
+    again = LT.TrajectoryPlan.from_circuit(n, gates, gamma, lam, count, tune=True)
+    assert again.config()[3] == 0.0 and again.config()[:3] == (threads, workgroups, fused)
+    plan.close()
+    again.close()
+
+
+def test_tuned_plan_mixed_batch():
+    from llsv import trajectory as LT
+
+    batch = C.mixed_batch(6, count=96, n_min=6, n_max=11, layers=4)
+    say("tuned plan mixed batch")
+    plan = LT.TrajectoryPlan(batch, tune=True)
+    z, j, h = plan.run(17)
+    zr, jr, hr = LT.run_batch(batch, seed=17)
+    assert np.array_equal(j, jr) and np.array_equal(h, hr)
+    assert np.max(np.abs(z - zr)) < 1e-12
+    plan.close()
