@@ -265,6 +265,46 @@ def trajectory_leg(n=14, count=4096, mixed=2048):
     }
 
 
+def latency_leg(V3, n=12, n_ops=1000, seeds=10):
+    """12 qubits, 1000 ops each (H on a random qubit or CNOT on a random ordered pair, p = 0.5;
+    seeds 0..9): latency-bound (the 64 KiB state fits one workgroup's LDS).  Per-gate calls
+    through the v3 surface (the reference's usage: one Python call per gate) and the whole
+    circuit as one program (one pass, one launch)."""
+    from llsv import circuits as C, gates as G
+
+    circuits = [C.random_h_cnot(s, n, n_ops) for s in range(seeds)]
+    H = G.matrix(("h", 0))
+
+    def per_gate(ops, sv):
+        for op in ops:
+            if op[0] == "h":
+                sv.apply_gate_cache_optimized(op[1], H)
+            else:
+                sv.cnot(op[1], op[2])
+        sv.sync()
+
+    sv = V3.StateVector(n)
+    per_gate(circuits[0], sv)
+    t0 = time.perf_counter()
+    for ops in circuits:
+        sv.reset()
+        per_gate(ops, sv)
+    t_gate = (time.perf_counter() - t0) / seeds
+    sv.apply_program(circuits[0])
+    sv.sync()
+    t0 = time.perf_counter()
+    for ops in circuits:
+        sv.reset()
+        sv.apply_program(ops)
+        sv.sync()
+    t_prog = (time.perf_counter() - t0) / seeds
+    log(f"latency {n}q x {n_ops} ops: per-gate {t_gate * 1e3:.2f} ms, program {t_prog * 1e3:.3f} ms per circuit")
+    del sv
+    return {"workload": f"{n}-qubit complex128, {n_ops} random H / CNOT ops (seeds 0..{seeds - 1})",
+            "per_gate_ms_per_circuit": round(t_gate * 1e3, 3), "per_gate_ops_per_s": round(n_ops / t_gate, 1),
+            "program_ms_per_circuit": round(t_prog * 1e3, 4), "program_ops_per_s": round(n_ops / t_prog, 1)}
+
+
 def program_leg(V3, n=28, layers=20):
     """Only the program path on the 28-qubit layered circuit (for counter runs)."""
     from llsv import circuits as C, program as P
@@ -346,7 +386,7 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the layered, adjoint, trajectory and 32-qubit legs")
-    ap.add_argument("--leg", choices=["layered28", "program28", "adjoint20", "trajectories14", "q32"],
+    ap.add_argument("--leg", choices=["latency12", "layered28", "program28", "adjoint20", "trajectories14", "q32"],
                     help="run only this leg (for profiling) and print its JSON")
     ap.add_argument("--probe", action="store_true",
                     help="launch each kernel a few times and exit (for rocprofv3 counter runs)")
@@ -364,7 +404,8 @@ def main():
     if a.leg:
         import torch  # noqa: F401
 
-        fn = {"layered28": lambda: layered_leg(V3), "program28": lambda: program_leg(V3),
+        fn = {"latency12": lambda: latency_leg(V3), "layered28": lambda: layered_leg(V3),
+              "program28": lambda: program_leg(V3),
               "adjoint20": adjoint_leg, "trajectories14": trajectory_leg,
               "q32": lambda: q32_leg(V3)}[a.leg]
         print(json.dumps({a.leg: fn()}), flush=True)
@@ -522,6 +563,7 @@ def main():
     }
     del sv
     if not a.no_extra:
+        line["latency12"] = latency_leg(V3)
         line["layered28"] = layered_leg(V3)
         line["adjoint20"] = adjoint_leg()
         line["trajectories14"] = trajectory_leg()

@@ -222,6 +222,23 @@ int llsv_pc_adjoint(int n_qubits, int batch, const llsv_pgate* gates, int n_gate
                     const double* params, const llsv_pauli* terms, int n_terms, double* energies,
                     double* grads, void* workspace, uint64_t workspace_bytes, void* stream);
 
+/* ---- readout ------------------------------------------------------------------ */
+
+/* <psi| sum_t terms[t].coeff P_t |psi> for Pauli strings P_t (llsv_pauli masks), in one pass
+ * per 16 terms with a fixed-shape reduction (bitwise reproducible).  per_term (may be NULL)
+ * receives the n_terms contributions, *total their sum.  Generalises the reference's
+ * Z-only readout (probabilities + NumPy, README:162) to any Pauli sum; no GPU counterpart
+ * in the reference. */
+int llsv_expect_pauli(llsv_state* s, const llsv_pauli* terms, int n_terms, double* per_term, double* total);
+
+/* Seeded measurement sampling in the computational basis: out[k] (host, shots entries) is the
+ * smallest basis index i with CDF(i) > u_k * CDF(last), CDF(i) = sum_{j <= i} |psi_j|^2 and
+ * u_k = (x >> 11) 2^-53, x the first 64 bits of Philox4x32-10(key = seed, counter = (k low,
+ * k high, 0x53414D50, 0)).  Indices with zero probability are never returned.  The state is
+ * unchanged.  No counterpart in the reference (its scripts sample from probabilities() in
+ * NumPy). */
+int llsv_sample(llsv_state* s, uint64_t shots, uint64_t seed, uint64_t* out);
+
 /* ---- batched noisy trajectories ---------------------------------------------- */
 
 /* A gate of a noisy circuit: after it, every qubit it touches goes through amplitude damping

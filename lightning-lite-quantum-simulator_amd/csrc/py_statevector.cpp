@@ -23,6 +23,7 @@
 #include <complex>
 #include <cstring>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/llsv.h"
@@ -162,6 +163,22 @@ public:
         device_call([&] { return llsv_expect_z(st_, q, &v); });
         return v;
     }
+    // <psi|H|psi> for H = sum of (coeff, xmask, zmask) Pauli terms
+    double expect_pauli(const std::vector<std::tuple<double, uint64_t, uint64_t>>& terms) {
+        std::vector<llsv_pauli> t(terms.size());
+        for (size_t i = 0; i < terms.size(); ++i) t[i] = llsv_pauli{std::get<0>(terms[i]), std::get<1>(terms[i]), std::get<2>(terms[i])};
+        double total = 0.0;
+        device_call([&] { return llsv_expect_pauli(st_, t.data(), int(t.size()), nullptr, &total); });
+        return total;
+    }
+
+    py::array_t<uint64_t> sample(uint64_t shots, uint64_t seed) {
+        py::array_t<uint64_t> out(static_cast<py::ssize_t>(shots));
+        uint64_t* p = out.mutable_data();
+        device_call([&] { return llsv_sample(st_, shots, seed, p); });
+        return out;
+    }
+
     py::array_t<double> probabilities() {
         py::array_t<double> out(static_cast<py::ssize_t>(size()));
         double* p = out.mutable_data();
@@ -371,6 +388,8 @@ PYBIND11_MODULE(LLSV_PYMOD, m) {
         .def("norm", &StateVector::norm)
         .def("expect_z", &StateVector::expect_z, py::arg("q"))
         .def("probabilities", &StateVector::probabilities)
+        .def("expect_pauli", &StateVector::expect_pauli, py::arg("terms"))
+        .def("sample", &StateVector::sample, py::arg("shots"), py::arg("seed"))
         .def("state", &StateVector::state)
         .def("sync", &StateVector::sync)
         .def("defer_host_sync", &StateVector::defer_host_sync, py::arg("on") = true)

@@ -45,6 +45,8 @@ SIGNATURES = {
     "llsv_norm2": (_I, [_P, ctypes.POINTER(_D)]),
     "llsv_expect_z": (_I, [_P, _I, ctypes.POINTER(_D)]),
     "llsv_probabilities": (_I, [_P, _P, _U64]),
+    "llsv_expect_pauli": (_I, [_P, _P, _I, _P, _P]),
+    "llsv_sample": (_I, [_P, _U64, _U64, _P]),
     "llsv_copy_to_host": (_I, [_P, _P, _U64]),
     "llsv_copy_from_host": (_I, [_P, _P, _U64]),
     "llsv_sync": (_I, [_P]),
