@@ -222,6 +222,14 @@ int llsv_pc_adjoint(int n_qubits, int batch, const llsv_pgate* gates, int n_gate
                     const double* params, const llsv_pauli* terms, int n_terms, double* energies,
                     double* grads, void* workspace, uint64_t workspace_bytes, void* stream);
 
+/* Compiles, without a device, the tiled sweep kernels llsv_pc_energy / llsv_pc_adjoint use for
+ * this gate list (n >= 20 in the automatic mode of llsv_set_program_jit, n >= 12 when it is 2):
+ * forward and backward passes of 12-qubit register tiles, so the first batched call does not
+ * wait for them.  Returns the number of compiled passes (0 when the sweeps run gate by gate),
+ * or an error code; *forward_passes (may be NULL) receives the forward sweep's share.
+ * No counterpart in the reference. */
+int llsv_pc_precompile(int n_qubits, const llsv_pgate* gates, int n_gates, int* forward_passes);
+
 /* ---- readout ------------------------------------------------------------------ */
 
 /* <psi| sum_t terms[t].coeff P_t |psi> for Pauli strings P_t (llsv_pauli masks), in one pass

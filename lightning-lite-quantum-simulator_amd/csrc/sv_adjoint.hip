@@ -11,13 +11,29 @@
 //   for k = G..1:  g_k = Im <lambda|G_k|psi>;  psi <- U_k^dag psi;  lambda <- U_k^dag lambda
 // The inner product of step k and both inverse gates are one pass over (psi, lambda).
 // Reductions use a fixed grid and fixed trees: gradients are bitwise reproducible.
+//
+// Tiled sweeps (n >= 12, when the pass compiler is on): the forward and backward sweeps are
+// grouped into register-resident passes of 12-qubit tiles (the program path's scheduler and
+// layouts), each compiled once per circuit structure.  A forward pass applies its gates to all
+// B states (grid.y = batch); a backward pass holds psi and lambda of a tile in registers and,
+// for each gate, reduces Im <lambda|G|psi> over the tile (one slot per gate) before applying
+// the inverse gate to both.  One-qubit coefficients (forward and inverse) come from a per-batch
+// table computed on the device from the parameters.  A 20-qubit, 590-gate circuit then costs
+// a few dozen passes per sweep instead of one per gate.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/llsv.h"
 #include "sv_common.h"
+#include "sv_jit.h"
+#include "sv_program.h"
 
 namespace llsv {
 
@@ -224,9 +240,12 @@ __global__ void k_pc_scatter(const double* __restrict__ slot_vals, const int* __
 
 struct Layout {
     uint64_t size;
-    size_t states, partials, slots, terms, slotmap, total;
+    size_t states, partials, slots, terms, slotmap, table, total;
     int nblk, nslot;
 };
+
+// Tiles of a tiled sweep (12-qubit register tiles), 0 when the state is smaller than a tile.
+uint64_t tiled_tiles(int n) { return n >= kRegTileBits ? uint64_t(1) << (n - kRegTileBits) : 0; }
 
 Layout layout_of(int n, int B, int n_gates, int nslot, int nterms) {
     Layout L{};
@@ -236,11 +255,13 @@ Layout layout_of(int n, int B, int n_gates, int nslot, int nterms) {
     L.nslot = nslot;
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
     L.states = al(L.size * 16 * size_t(B));
-    L.partials = al(sizeof(double) * size_t(B) * (size_t(n_gates) + 1) * kRedGrid);
+    const uint64_t per_slot = std::max<uint64_t>(kRedGrid, tiled_tiles(n));
+    L.partials = al(sizeof(double) * size_t(B) * (size_t(n_gates) + 1) * per_slot);
     L.slots = al(sizeof(double) * size_t(B) * (size_t(n_gates) + 1));
     L.terms = al(sizeof(PauliDev) * size_t(nterms > 0 ? nterms : 1));
     L.slotmap = al(sizeof(int) * size_t(n_gates + 1));
-    L.total = 2 * L.states + L.partials + L.slots + L.terms + L.slotmap;
+    L.table = al(sizeof(double) * 16 * size_t(B) * size_t(n_gates + 1));  // forward + inverse 2x2 per one-qubit gate
+    L.total = 2 * L.states + L.partials + L.slots + L.terms + L.slotmap + L.table;
     return L;
 }
 
@@ -295,6 +316,179 @@ hipError_t apply_gate(hipStream_t st, v2d* A, v2d* Lm, bool two, const Layout& L
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- tiled sweeps
+// table[b * tstride + 8 * j] = forward 2x2 of one-qubit gate coef_gate[j] for batch element b,
+// table[b * tstride + 8 * (ncoef + j)] = its inverse.
+__global__ __launch_bounds__(256) void k_pc_table(const PG* __restrict__ gates, const int* __restrict__ coef_gate,
+                                                  int ncoef, const double* __restrict__ params, int P, int B,
+                                                  double* __restrict__ table, uint64_t tstride) {
+    const int j = blockIdx.x * 256 + threadIdx.x, b = blockIdx.y;
+    if (j >= ncoef || b >= B) return;
+    const PG g = gates[coef_gate[j]];
+    double* out = table + uint64_t(b) * tstride;
+    gate_matrix(g, params, P, b, false, out + 8 * j);
+    gate_matrix(g, params, P, b, true, out + 8 * (ncoef + j));
+}
+
+struct TiledPass {
+    PassDesc d;
+    JitEntry* e = nullptr;
+};
+
+struct TiledPlan {
+    int ncoef = 0;
+    std::vector<int> coef_gate;  // gate index of coefficient j
+    std::vector<TiledPass> fwd, bwd;
+};
+
+// Schedule of the gate list as register-tile passes: ops in execution order; a one-qubit gate
+// becomes P_1QT on its coefficient (the inverse one in the backward sweep, preceded there by
+// P_GRAD for a rotation), CNOT and CZ as in the program path.
+std::vector<TiledPass> tiled_passes(int n, const std::vector<const llsv_pgate*>& seq, const std::vector<int>& coef_of,
+                                    const std::vector<int>& slot_of, const std::vector<int>& gate_index, int ncoef,
+                                    bool backward) {
+    const int G = int(seq.size());
+    std::vector<llsv_op> ops(G);
+    for (int k = 0; k < G; ++k) {
+        const llsv_pgate& g = *seq[k];
+        llsv_op& o = ops[k];
+        o = llsv_op{};
+        if (g.kind >= LLSV_PG_CNOT) {
+            o.kind = g.kind == LLSV_PG_CNOT ? LLSV_OP_CNOT : LLSV_OP_CZ;
+            o.nq = 2;
+            o.q[0] = g.q0;
+            o.q[1] = g.q1;
+        } else {
+            o.kind = LLSV_OP_1Q;  // scheduled as a dense 2x2 (the values live in the table)
+            o.nq = 1;
+            o.q[0] = g.q0;
+            for (int i = 0; i < 8; ++i) o.m[i] = 1.0;
+        }
+    }
+    const int b = kRegTileBits, L = 3;
+    std::vector<uint64_t> order(static_cast<size_t>(G)), pass_end(static_cast<size_t>(G) + 1);
+    const int np = plan_order(n, ops.data(), uint64_t(G), b, L, order.data(), pass_end.data());
+    std::vector<TiledPass> out;
+    uint64_t first = 0;
+    for (int p = 0; p < np; ++p) {
+        const uint64_t last = pass_end[p];
+        uint64_t T = (uint64_t(1) << L) - 1;
+        for (uint64_t i = first; i < last; ++i) {
+            const llsv_op& o = ops[order[i]];
+            if (o.kind == LLSV_OP_1Q) T |= uint64_t(1) << o.q[0];
+            if (o.kind == LLSV_OP_CNOT) T |= (uint64_t(1) << o.q[0]) | (uint64_t(1) << o.q[1]);
+        }
+        for (int q = 0; q < n && __builtin_popcountll(T) < b; ++q) T |= uint64_t(1) << q;
+        TiledPass tp;
+        PassDesc& d = tp.d;
+        d.n = n;
+        d.b = b;
+        d.L = L;
+        int local_of[64];
+        for (int q = 0; q < 64; ++q) local_of[q] = -1;
+        int j = 0;
+        for (int q = 0; q < L; ++q) d.tq[j++] = q;
+        for (int q = L; q < n; ++q)
+            if ((T >> q) & 1) d.tq[j++] = q;
+        for (int t = 0; t < b; ++t) {
+            local_of[d.tq[t]] = t;
+            d.pos[t] = d.tq[t];
+        }
+        std::sort(d.pos, d.pos + b);
+        std::vector<PassOp> pops;
+        for (uint64_t i = first; i < last; ++i) {
+            const int k = int(order[i]);
+            const llsv_pgate& g = *seq[k];
+            PassOp po{};
+            if (g.kind == LLSV_PG_CNOT) {
+                po.kind = P_CNOT;
+                po.a = local_of[g.q0];
+                po.b = local_of[g.q1];
+                pops.push_back(po);
+            } else if (g.kind == LLSV_PG_CZ) {
+                po.kind = P_MCZ;
+                for (int q : {g.q0, g.q1}) {
+                    if (local_of[q] >= 0) po.lmask |= uint64_t(1) << local_of[q];
+                    else po.omask |= uint64_t(1) << q;
+                }
+                pops.push_back(po);
+            } else {
+                const int gi = gate_index[k];
+                if (backward && g.kind <= LLSV_PG_RZ) {
+                    PassOp gr{};
+                    gr.kind = P_GRAD;
+                    gr.a = local_of[g.q0];
+                    gr.b = g.kind == LLSV_PG_RX ? 0 : g.kind == LLSV_PG_RY ? 1 : 2;
+                    gr.moff = slot_of[gi];
+                    pops.push_back(gr);
+                }
+                po.kind = P_1QT;
+                po.a = local_of[g.q0];
+                po.moff = coef_of[gi] + (backward ? ncoef : 0);
+                pops.push_back(po);
+            }
+        }
+        const std::vector<RegOp> rops = to_register_program(pops, L, d.rq);
+        d.nops = int(rops.size());
+        tp.e = jit_entry(jit_adjoint_source(rops.data(), int(rops.size()), backward));
+        out.push_back(tp);
+        first = last;
+    }
+    return out;
+}
+
+std::mutex g_tiled_mu;
+std::map<std::string, std::unique_ptr<TiledPlan>>& tiled_cache() {
+    static auto* m = new std::map<std::string, std::unique_ptr<TiledPlan>>();  // lives to process end
+    return *m;
+}
+
+// The tiled plan of a circuit (built and cached on first sight), or nullptr when the sweeps run
+// gate by gate: state below one tile, pass compiler off, or kernels not compiled (yet).
+TiledPlan* tiled_plan(int n, const llsv_pgate* gates, int G, const std::vector<int>& slot_of) {
+    if (n < kRegTileBits || G == 0 || jit_mode() == kJitOff) return nullptr;
+    if (!(jit_mode() == kJitAlways || n >= kJitMinQubits)) return nullptr;
+    std::string key = std::to_string(n) + ":";
+    for (int k = 0; k < G; ++k) {
+        const llsv_pgate& g = gates[k];
+        key += std::to_string(g.kind) + "," + std::to_string(g.q0) + "," + std::to_string(g.kind >= LLSV_PG_CNOT ? g.q1 : -1) + ";";
+    }
+    TiledPlan* tp = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_tiled_mu);
+        auto it = tiled_cache().find(key);
+        if (it == tiled_cache().end()) {
+            auto plan = std::make_unique<TiledPlan>();
+            std::vector<int> coef_of(G, -1), idx(G);
+            for (int k = 0; k < G; ++k)
+                if (gates[k].kind <= LLSV_PG_1Q) {
+                    coef_of[k] = plan->ncoef++;
+                    plan->coef_gate.push_back(k);
+                }
+            std::vector<const llsv_pgate*> seq(G);
+            for (int k = 0; k < G; ++k) {
+                seq[k] = &gates[k];
+                idx[k] = k;
+            }
+            plan->fwd = tiled_passes(n, seq, coef_of, slot_of, idx, plan->ncoef, false);
+            std::reverse(seq.begin(), seq.end());
+            std::reverse(idx.begin(), idx.end());
+            plan->bwd = tiled_passes(n, seq, coef_of, slot_of, idx, plan->ncoef, true);
+            it = tiled_cache().emplace(key, std::move(plan)).first;
+        }
+        tp = it->second.get();
+    }
+    std::vector<JitEntry*> want;
+    for (const auto* v : {&tp->fwd, &tp->bwd})
+        for (const TiledPass& p : *v)
+            if (!jit_ready(p.e)) want.push_back(p.e);
+    if (!want.empty()) jit_request(want, jit_mode() == kJitAlways);
+    for (const auto* v : {&tp->fwd, &tp->bwd})
+        for (const TiledPass& p : *v)
+            if (!jit_ready(p.e)) return nullptr;
+    return tp;
+}
+
 int run(int n, int B, const llsv_pgate* gates, int G, int P, const double* params, const llsv_pauli* terms, int T,
         double* energies, double* grads, void* ws, uint64_t ws_bytes, void* stream, const char* who) {
     if (int rc = validate(n, B, gates, G, P, params, terms, T, who)) return rc;
@@ -338,9 +532,33 @@ int run(int n, int B, const llsv_pgate* gates, int G, int P, const double* param
     if (adj && !slot_param.empty())
         CK(hipMemcpyAsync(dslotmap, slot_param.data(), sizeof(int) * slot_param.size(), hipMemcpyHostToDevice, st));
 
+    off += L.slotmap;
+    double* table = reinterpret_cast<double*>(base + off);
+    TiledPlan* tiled = tiled_plan(n, gates, G, slot_of);
+    const uint64_t tstride = 16 * uint64_t(tiled ? tiled->ncoef : 0);
+    const int ntiles = int(tiled_tiles(n));
+    if (tiled && tiled->ncoef) {
+        std::vector<PG> hg(G);
+        std::memcpy(hg.data(), gates, sizeof(PG) * size_t(G));
+        // gates and the coefficient map go through the (not yet used) partials area
+        PG* dgates = reinterpret_cast<PG*>(partials);
+        int* dcoef = reinterpret_cast<int*>(dgates + G);
+        CK(hipMemcpyAsync(dgates, hg.data(), sizeof(PG) * size_t(G), hipMemcpyHostToDevice, st));
+        CK(hipMemcpyAsync(dcoef, tiled->coef_gate.data(), sizeof(int) * tiled->coef_gate.size(), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_pc_table, dim3(unsigned((tiled->ncoef + 255) / 256), unsigned(B)), dim3(256), 0, st, dgates,
+                           dcoef, tiled->ncoef, params, P, B, table, tstride);
+        CK(hipGetLastError());
+    }
+
     hipLaunchKernelGGL(k_pc_init, dim3(unsigned((L.size + 255) / 256), unsigned(B)), dim3(256), 0, st, A, L.size);
     CK(hipGetLastError());
-    for (int k = 0; k < G; ++k) CK(apply_gate(st, A, nullptr, false, L, B, gates[k], params, P, false, partials, -1));
+    if (tiled) {
+        for (const TiledPass& p : tiled->fwd)
+            CK(jit_launch_batched(p.e, unsigned(ntiles), unsigned(B), st, A, nullptr, L.size, p.d, table, tstride, nullptr,
+                                  0, ntiles));
+    } else {
+        for (int k = 0; k < G; ++k) CK(apply_gate(st, A, nullptr, false, L, B, gates[k], params, P, false, partials, -1));
+    }
 
     const dim3 rgrid(unsigned(L.nblk), unsigned(B));
     if (adj)
@@ -352,10 +570,18 @@ int run(int n, int B, const llsv_pgate* gates, int G, int P, const double* param
     CK(hipGetLastError());
     if (!adj) return LLSV_OK;
 
-    for (int k = G - 1; k >= 0; --k)
-        CK(apply_gate(st, A, Lm, true, L, B, gates[k], params, P, true, partials, slot_of[k]));
+    int per_slot = L.nblk;
+    if (tiled) {
+        for (const TiledPass& p : tiled->bwd)
+            CK(jit_launch_batched(p.e, unsigned(ntiles), unsigned(B), st, A, Lm, L.size, p.d, table, tstride, partials,
+                                  L.nslot, ntiles));
+        per_slot = ntiles;
+    } else {
+        for (int k = G - 1; k >= 0; --k)
+            CK(apply_gate(st, A, Lm, true, L, B, gates[k], params, P, true, partials, slot_of[k]));
+    }
     if (!slot_param.empty()) {
-        hipLaunchKernelGGL(k_pc_reduce, dim3(unsigned(L.nslot), unsigned(B)), dim3(256), 0, st, partials, L.nblk,
+        hipLaunchKernelGGL(k_pc_reduce, dim3(unsigned(L.nslot), unsigned(B)), dim3(256), 0, st, partials, per_slot,
                            L.nslot, slots);
         CK(hipGetLastError());
     }
@@ -380,6 +606,28 @@ int llsv_pc_energy(int n, int batch, const llsv_pgate* gates, int n_gates, int n
                    const llsv_pauli* terms, int n_terms, double* energies, void* ws, uint64_t ws_bytes, void* stream) {
     return llsv::run(n, batch, gates, n_gates, n_params, params, terms, n_terms, energies, nullptr, ws, ws_bytes,
                      stream, "llsv_pc_energy");
+}
+
+int llsv_pc_precompile(int n, const llsv_pgate* gates, int n_gates, int* forward_passes) {
+    using namespace llsv;
+    if (n < 1 || n > 30 || n_gates < 0 || (n_gates && !gates)) return fail(LLSV_ERR_ARG, "llsv_pc_precompile: bad arguments");
+    std::vector<int> slot_of(size_t(n_gates), -1);
+    int ns = 0;
+    for (int k = 0; k < n_gates; ++k) {
+        if (gates[k].kind < LLSV_PG_RX || gates[k].kind > LLSV_PG_CZ || gates[k].q0 < 0 || gates[k].q0 >= n ||
+            (gates[k].kind >= LLSV_PG_CNOT && (gates[k].q1 < 0 || gates[k].q1 >= n || gates[k].q1 == gates[k].q0)))
+            return fail(LLSV_ERR_ARG, "llsv_pc_precompile: gate %d: bad kind or qubits", k);
+        if (gates[k].kind <= LLSV_PG_RZ) slot_of[k] = ns++;
+    }
+    if (forward_passes) *forward_passes = 0;
+    const int prev = jit_mode();
+    if (prev == kJitOff || (prev == kJitAuto && n < kJitMinQubits)) return 0;
+    set_jit_mode(kJitAlways);  // compile now, in this thread
+    TiledPlan* tp = tiled_plan(n, gates, n_gates, slot_of);
+    set_jit_mode(prev);
+    if (!tp) return 0;
+    if (forward_passes) *forward_passes = int(tp->fwd.size());
+    return int(tp->fwd.size() + tp->bwd.size());
 }
 
 int llsv_pc_adjoint(int n, int batch, const llsv_pgate* gates, int n_gates, int n_params, const double* params,

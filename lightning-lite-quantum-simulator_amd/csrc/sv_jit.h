@@ -40,6 +40,11 @@ bool jit_wanted(int n, int nops, bool dense);
 // batch renames it and prepends the shared prelude).
 std::string jit_source(const RegOp* ops, int nops);
 
+// Source of a batched pass of a parameterised circuit (states A + b * size for b = blockIdx.y,
+// one-qubit coefficients from the table T + b * tstride).  backward: lambda goes through every
+// op too, and the RO_GRAD ops' per-tile sums land in partials[(b * nslot + slot) * ntiles + tile].
+std::string jit_adjoint_source(const RegOp* ops, int nops, bool backward);
+
 struct JitEntry;
 
 // The cache entry for a kernel source (created on first sight, not compiled).
@@ -56,6 +61,11 @@ void jit_wait();
 
 // Launches a ready entry's kernel on the current device: grid workgroups of 256 threads.
 hipError_t jit_launch(JitEntry* e, unsigned grid, hipStream_t st, v2d* psi, const PassDesc& d, const RegOp* dops);
+
+// Launches a ready batched entry (jit_adjoint_source) on grid (tiles, batch).
+hipError_t jit_launch_batched(JitEntry* e, unsigned tiles, unsigned batch, hipStream_t st, v2d* A, v2d* Lm,
+                              uint64_t size, const PassDesc& d, const double* T, uint64_t tstride, double* partials,
+                              int nslot, int ntiles);
 
 void jit_stats(uint64_t* compiled, uint64_t* launches, uint64_t* failures);
 

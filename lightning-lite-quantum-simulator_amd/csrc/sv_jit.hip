@@ -77,13 +77,36 @@ bool jit_wanted(int n, int nops, bool dense) {
     return m == kJitAlways || n >= kJitMinQubits;
 }
 
-std::string jit_source(const RegOp* ops, int nops) {
+namespace {
+
+// What a generated kernel looks like around the op sequence.
+struct Shape {
+    bool batched = false;  // blockIdx.y = batch element: states A + b * size, coefficients T + b * tstride
+    bool two = false;      // a second state (lambda) goes through every op; RO_GRAD ops reduce into partials
+};
+
+std::string emit(const RegOp* ops, int nops, const Shape& sh) {
     const size_t stride = sizeof(RegOp) / sizeof(double), moff = offsetof(RegOp, m) / sizeof(double);
+    const char* arrs[2] = {"a", "l"};
+    const int na = sh.two ? 2 : 1;
+    std::vector<int> gslot;  // gradient slots of this pass, in order of appearance
+    for (int i = 0; i < nops; ++i)
+        if (ops[i].kind == RO_GRAD) gslot.push_back(ops[i].moff);
     Src o;
-    o.s.reserve(4096 + size_t(nops) * 600);
-    o.line("extern \"C\" __global__ __launch_bounds__(256) void LLSV_KERNEL(v2d* __restrict__ psi, PassDesc d, "
-           "const double* __restrict__ M) {");
+    o.s.reserve(4096 + size_t(nops) * 600 * na);
+    if (!sh.batched) {
+        o.line("extern \"C\" __global__ __launch_bounds__(256) void LLSV_KERNEL(v2d* __restrict__ psi, PassDesc d, "
+               "const double* __restrict__ M) {");
+    } else {
+        o.line("extern \"C\" __global__ __launch_bounds__(256) void LLSV_KERNEL(v2d* __restrict__ A, v2d* __restrict__ Lm, "
+               "unsigned long long size, PassDesc d, const double* __restrict__ T, unsigned long long tstride, "
+               "double* __restrict__ partials, int nslot, int ntiles) {");
+        o.line("v2d* const psi = A + (unsigned long long)blockIdx.y * size;");
+        if (sh.two) o.line("v2d* const lam = Lm + (unsigned long long)blockIdx.y * size;");
+        o.line("const double* const M = T + (unsigned long long)blockIdx.y * tstride;");
+    }
     o.line("__shared__ v2d tile[4096];");
+    if (!gslot.empty()) o.line("__shared__ double gs[%zu][4];", gslot.size());
     o.line("char* const lds = (char*)tile;");
     o.line("const unsigned int tid = threadIdx.x;");
     o.line("unsigned long long base = blockIdx.x;");
@@ -91,67 +114,102 @@ std::string jit_source(const RegOp* ops, int nops) {
     o.line("unsigned long long lg[12];");
     o.line("#pragma unroll");
     o.line("for (int j = 0; j < 12; ++j) lg[j] = j < d.L ? (1ull << j) : (1ull << d.tq[j]);");
-    o.line("v2d a[16];");
+    for (int k = 0; k < na; ++k) o.line("v2d %s[16];", arrs[k]);
     o.line("{ unsigned long long gt = 0;");
     o.line("#pragma unroll");
     o.line("for (int j = 0; j < 8; ++j) if ((tid >> j) & 1u) gt |= lg[j];");
     o.line("#pragma unroll");
-    o.line("for (int j = 0; j < 16; ++j) a[j] = __builtin_nontemporal_load(psi + (base | gt | ((j & 1) ? lg[8] : 0ull) | "
-           "((j & 2) ? lg[9] : 0ull) | ((j & 4) ? lg[10] : 0ull) | ((j & 8) ? lg[11] : 0ull))); }");
+    o.line("for (int j = 0; j < 16; ++j) { const unsigned long long o = base | gt | ((j & 1) ? lg[8] : 0ull) | "
+           "((j & 2) ? lg[9] : 0ull) | ((j & 4) ? lg[10] : 0ull) | ((j & 8) ? lg[11] : 0ull);");
+    o.line("a[j] = __builtin_nontemporal_load(psi + o);%s } }", sh.two ? " l[j] = __builtin_nontemporal_load(lam + o);" : "");
     o.line("unsigned int lt = tid, sl = r_swz(tid) << 4;");
     o.line("double sr = 1.0, si = 0.0;");
 
     uint32_t S = 0xF00u;  // register bits of the current layout (the kernel starts with 8..11)
     bool scaled = false;
+    int gi = 0;
     for (int i = 0; i < nops; ++i) {
         const RegOp& op = ops[i];
         const size_t c = size_t(i) * stride + moff;
         switch (op.kind) {
-            case RO_1Q: {
+            case RO_1Q:
+            case RO_1QT: {
                 const int K = 1 << op.a;
-                o.line("{ const double* m = M + %zu;", c);
-                for (int j = 0; j < 16; ++j)
-                    if (!(j & K)) o.line("r_pu(a[%d], a[%d], m);", j, j | K);
+                if (op.kind == RO_1Q)
+                    o.line("{ const double* m = M + %zu;", c);
+                else
+                    o.line("{ const double* m = M + %zu;", size_t(8) * size_t(op.moff));
+                for (int k = 0; k < na; ++k)
+                    for (int j = 0; j < 16; ++j)
+                        if (!(j & K)) o.line("r_pu(%s[%d], %s[%d], m);", arrs[k], j, arrs[k], j | K);
                 o.line("}");
+                break;
+            }
+            case RO_GRAD: {
+                // Im <lambda| G |psi> over this lane's pairs along register bit a (G = X, Y or Z on it)
+                const int K = 1 << op.a;
+                o.line("{ double g = 0.0;");
+                for (int j = 0; j < 16; ++j) {
+                    if (j & K) continue;
+                    const int u = j | K;
+                    if (op.q[0] == 2)
+                        o.line("g += (l[%d].x * a[%d].y - l[%d].y * a[%d].x) - (l[%d].x * a[%d].y - l[%d].y * a[%d].x);",
+                               j, j, j, j, u, u, u, u);
+                    else if (op.q[0] == 0)
+                        o.line("g += (l[%d].x * a[%d].y - l[%d].y * a[%d].x) + (l[%d].x * a[%d].y - l[%d].y * a[%d].x);",
+                               j, u, j, u, u, j, u, j);
+                    else
+                        o.line("g += (l[%d].x * a[%d].x + l[%d].y * a[%d].y) - (l[%d].x * a[%d].x + l[%d].y * a[%d].y);",
+                               u, j, u, j, j, u, j, u);
+                }
+                o.line("#pragma unroll");
+                o.line("for (int o = 32; o > 0; o >>= 1) g += __shfl_xor(g, o, 64);");
+                o.line("if ((tid & 63u) == 0u) gs[%d][tid >> 6] = g; }", gi);
+                ++gi;
                 break;
             }
             case RO_CNOT: {
                 const int K = 1 << op.a;
                 if (op.tmask == 0) {
-                    for (int j = 0; j < 16; ++j)
-                        if (!(j & K) && ((op.rmask >> j) & 1u))
-                            o.line("{ const v2d t = a[%d]; a[%d] = a[%d]; a[%d] = t; }", j, j, j | K, j | K);
+                    for (int k = 0; k < na; ++k)
+                        for (int j = 0; j < 16; ++j)
+                            if (!(j & K) && ((op.rmask >> j) & 1u))
+                                o.line("{ const v2d t = %s[%d]; %s[%d] = %s[%d]; %s[%d] = t; }", arrs[k], j, arrs[k], j,
+                                       arrs[k], j | K, arrs[k], j | K);
                 } else {
                     o.line("{ const bool tk = (lt & %uu) != 0u;", op.tmask);
-                    for (int j = 0; j < 16; ++j)
-                        if (!(j & K))
-                            o.line("{ const v2d x = a[%d], y = a[%d]; a[%d] = tk ? y : x; a[%d] = tk ? x : y; }", j, j | K,
-                                   j, j | K);
+                    for (int k = 0; k < na; ++k)
+                        for (int j = 0; j < 16; ++j)
+                            if (!(j & K))
+                                o.line("{ const v2d x = %s[%d], y = %s[%d]; %s[%d] = tk ? y : x; %s[%d] = tk ? x : y; }",
+                                       arrs[k], j, arrs[k], j | K, arrs[k], j, arrs[k], j | K);
                     o.line("}");
                 }
                 break;
             }
             case RO_DIAG: {
                 if (op.tmask == 0) {
-                    for (int j = 0; j < 16; ++j) {
-                        if ((op.rmask >> j) & 1u)
-                            o.line("r_sc(a[%d], M[%zu], M[%zu]);", j, c + 6, c + 7);
-                        else if (!op.a)
-                            o.line("r_sc(a[%d], M[%zu], M[%zu]);", j, c + 0, c + 1);
-                    }
+                    for (int k = 0; k < na; ++k)
+                        for (int j = 0; j < 16; ++j) {
+                            if ((op.rmask >> j) & 1u)
+                                o.line("r_sc(%s[%d], M[%zu], M[%zu]);", arrs[k], j, c + 6, c + 7);
+                            else if (!op.a)
+                                o.line("r_sc(%s[%d], M[%zu], M[%zu]);", arrs[k], j, c + 0, c + 1);
+                        }
                 } else {
                     o.line("{ const bool hi = (lt & %uu) != 0u;", op.tmask);
                     o.line("const double dr = hi ? M[%zu] : M[%zu], di = hi ? M[%zu] : M[%zu];", c + 6, c, c + 7, c + 1);
                     o.line("#pragma unroll");
-                    o.line("for (int j = 0; j < 16; ++j) r_sc(a[j], dr, di); }");
+                    o.line("for (int j = 0; j < 16; ++j) { r_sc(a[j], dr, di);%s } }", sh.two ? " r_sc(l[j], dr, di);" : "");
                 }
                 break;
             }
             case RO_MCZ: {
                 if (!op.rmask) break;
                 if (op.omask == 0 && op.tmask == 0) {
-                    for (int j = 0; j < 16; ++j)
-                        if ((op.rmask >> j) & 1u) o.line("a[%d] = -a[%d];", j, j);
+                    for (int k = 0; k < na; ++k)
+                        for (int j = 0; j < 16; ++j)
+                            if ((op.rmask >> j) & 1u) o.line("%s[%d] = -%s[%d];", arrs[k], j, arrs[k], j);
                     break;
                 }
                 std::string cond;
@@ -166,8 +224,9 @@ std::string jit_source(const RegOp* ops, int nops) {
                     cond += buf;
                 }
                 o.line("{ const unsigned int sg = (%s) ? 0x80000000u : 0u;", cond.c_str());
-                for (int j = 0; j < 16; ++j)
-                    if ((op.rmask >> j) & 1u) o.line("a[%d] = r_ng(a[%d], sg);", j, j);
+                for (int k = 0; k < na; ++k)
+                    for (int j = 0; j < 16; ++j)
+                        if ((op.rmask >> j) & 1u) o.line("%s[%d] = r_ng(%s[%d], sg);", arrs[k], j, arrs[k], j);
                 o.line("}");
                 break;
             }
@@ -177,14 +236,18 @@ std::string jit_source(const RegOp* ops, int nops) {
                        op.a, c + 6, c, c + 7, c + 1);
                 break;
             case RO_LAYOUT: {
-                o.line("{");
-                for (int j = 0; j < 16; ++j) o.line("*(v2d*)(lds + (sl ^ %uu)) = a[%d];", op.off0[j], j);
-                o.line("__syncthreads();");
-                o.line("lt = r_ins0(r_ins0(r_ins0(r_ins0(tid, %d), %d), %d), %d);", op.q[0], op.q[1], op.q[2], op.q[3]);
-                o.line("sl = r_swz(lt) << 4;");
-                for (int j = 0; j < 16; ++j) o.line("a[%d] = *(const v2d*)(lds + (sl ^ %uu));", j, op.off1[j]);
-                o.line("__syncthreads();");
-                o.line("}");
+                // one state at a time through the 64 KiB exchange buffer
+                o.line("{ const unsigned int nl = r_ins0(r_ins0(r_ins0(r_ins0(tid, %d), %d), %d), %d);", op.q[0], op.q[1],
+                       op.q[2], op.q[3]);
+                o.line("const unsigned int nsl = r_swz(nl) << 4;");
+                for (int k = 0; k < na; ++k) {
+                    if (k) o.line("__syncthreads();");
+                    for (int j = 0; j < 16; ++j) o.line("*(v2d*)(lds + (sl ^ %uu)) = %s[%d];", op.off0[j], arrs[k], j);
+                    o.line("__syncthreads();");
+                    for (int j = 0; j < 16; ++j) o.line("%s[%d] = *(const v2d*)(lds + (nsl ^ %uu));", arrs[k], j, op.off1[j]);
+                }
+                o.line("lt = nl; sl = nsl;");
+                o.line("__syncthreads(); }");
                 S = (1u << op.q[0]) | (1u << op.q[1]) | (1u << op.q[2]) | (1u << op.q[3]);
                 break;
             }
@@ -202,14 +265,35 @@ std::string jit_source(const RegOp* ops, int nops) {
     if (scaled) {
         o.line("if (sr != 1.0 || si != 0.0) {");
         o.line("#pragma unroll");
-        o.line("for (int j = 0; j < 16; ++j) r_sc(a[j], sr, si); }");
+        o.line("for (int j = 0; j < 16; ++j) { r_sc(a[j], sr, si);%s } }", sh.two ? " r_sc(l[j], sr, si);" : "");
     }
     o.line("#pragma unroll");
-    o.line("for (int j = 0; j < 16; ++j) __builtin_nontemporal_store(a[j], psi + (base | gt | ((j & 1) ? lg[%d] : 0ull) | "
-           "((j & 2) ? lg[%d] : 0ull) | ((j & 4) ? lg[%d] : 0ull) | ((j & 8) ? lg[%d] : 0ull))); }",
+    o.line("for (int j = 0; j < 16; ++j) { const unsigned long long o = base | gt | ((j & 1) ? lg[%d] : 0ull) | "
+           "((j & 2) ? lg[%d] : 0ull) | ((j & 4) ? lg[%d] : 0ull) | ((j & 8) ? lg[%d] : 0ull);",
            rq[0], rq[1], rq[2], rq[3]);
+    o.line("__builtin_nontemporal_store(a[j], psi + o);%s } }", sh.two ? " __builtin_nontemporal_store(l[j], lam + o);" : "");
+    if (!gslot.empty()) {
+        std::string list;
+        for (size_t k = 0; k < gslot.size(); ++k) list += (k ? ", " : "") + std::to_string(gslot[k]);
+        o.line("{ const int gslot[%zu] = {%s};", gslot.size(), list.c_str());
+        o.line("__syncthreads();");
+        o.line("for (unsigned int k = tid; k < %zuu; k += 256u)", gslot.size());
+        o.line("partials[((unsigned long long)blockIdx.y * nslot + gslot[k]) * ntiles + blockIdx.x] = "
+               "(gs[k][0] + gs[k][1]) + (gs[k][2] + gs[k][3]); }");
+    }
     o.line("}");
     return o.s;
+}
+
+}  // namespace
+
+std::string jit_source(const RegOp* ops, int nops) { return emit(ops, nops, Shape{}); }
+
+std::string jit_adjoint_source(const RegOp* ops, int nops, bool backward) {
+    Shape sh;
+    sh.batched = true;
+    sh.two = backward;
+    return emit(ops, nops, sh);
 }
 
 // A code object holds the kernels of one compile batch; modules are loaded per device on first launch.
@@ -392,34 +476,59 @@ void jit_wait() {
     g_cv.wait(lock, [] { return g_queue.empty() && g_busy == 0; });
 }
 
-hipError_t jit_launch(JitEntry* e, unsigned grid, hipStream_t st, v2d* psi, const PassDesc& d, const RegOp* dops) {
+namespace {
+
+// The entry's kernel on the current device (its module is loaded on first use).
+hipError_t entry_function(JitEntry* e, hipFunction_t* out) {
     int dev = 0;
     hipError_t err = hipGetDevice(&dev);
     if (err != hipSuccess) return err;
-    hipFunction_t fn = nullptr;
-    {
-        std::lock_guard<std::mutex> lock(g_mu);
-        auto it = e->fn.find(dev);
-        if (it == e->fn.end()) {
-            CodeObject& obj = *e->obj;
-            auto mi = obj.mod.find(dev);
-            if (mi == obj.mod.end()) {
-                hipModule_t mod = nullptr;
-                err = hipModuleLoadData(&mod, obj.code.data());
-                if (err != hipSuccess) return err;
-                mi = obj.mod.emplace(dev, mod).first;
-            }
-            err = hipModuleGetFunction(&fn, mi->second, e->name.c_str());
-            if (err != hipSuccess) return err;
-            e->fn[dev] = fn;
-        } else {
-            fn = it->second;
-        }
+    std::lock_guard<std::mutex> lock(g_mu);
+    auto it = e->fn.find(dev);
+    if (it != e->fn.end()) {
+        *out = it->second;
+        return hipSuccess;
     }
+    CodeObject& obj = *e->obj;
+    auto mi = obj.mod.find(dev);
+    if (mi == obj.mod.end()) {
+        hipModule_t mod = nullptr;
+        err = hipModuleLoadData(&mod, obj.code.data());
+        if (err != hipSuccess) return err;
+        mi = obj.mod.emplace(dev, mod).first;
+    }
+    hipFunction_t fn = nullptr;
+    err = hipModuleGetFunction(&fn, mi->second, e->name.c_str());
+    if (err != hipSuccess) return err;
+    e->fn[dev] = fn;
+    *out = fn;
+    return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t jit_launch(JitEntry* e, unsigned grid, hipStream_t st, v2d* psi, const PassDesc& d, const RegOp* dops) {
+    hipFunction_t fn = nullptr;
+    hipError_t err = entry_function(e, &fn);
+    if (err != hipSuccess) return err;
     PassDesc dd = d;
     const double* M = reinterpret_cast<const double*>(dops);
     void* args[] = {&psi, &dd, &M};
     err = hipModuleLaunchKernel(fn, grid, 1, 1, 256, 1, 1, 0, st, args, nullptr);
+    if (err == hipSuccess) ++g_launches;
+    return err;
+}
+
+hipError_t jit_launch_batched(JitEntry* e, unsigned tiles, unsigned batch, hipStream_t st, v2d* A, v2d* Lm,
+                              uint64_t size, const PassDesc& d, const double* T, uint64_t tstride, double* partials,
+                              int nslot, int ntiles) {
+    hipFunction_t fn = nullptr;
+    hipError_t err = entry_function(e, &fn);
+    if (err != hipSuccess) return err;
+    PassDesc dd = d;
+    unsigned long long sz = size, ts = tstride;
+    void* args[] = {&A, &Lm, &sz, &dd, &T, &ts, &partials, &nslot, &ntiles};
+    err = hipModuleLaunchKernel(fn, tiles, batch, 1, 256, 1, 1, 0, st, args, nullptr);
     if (err == hipSuccess) ++g_launches;
     return err;
 }

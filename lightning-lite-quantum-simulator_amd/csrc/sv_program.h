@@ -30,6 +30,10 @@ enum PassOpKind : int {
     P_CNOT = 3,    // a = local control, b = local target
     P_MCZ = 4,     // negate where (l & lmask) == lmask, if (tile base & omask) == omask
     P_DENSE = 5,   // a = k, q[0..k) local bits, moff = offset (in v2d) of the matrix
+    // batched parameterised circuits (register-resident passes only):
+    P_1QT = 6,     // a = local bit, moff = index of the 2x2 in the per-batch coefficient table
+    P_GRAD = 7,    // a = local bit, b = generator (0 X, 1 Y, 2 Z), moff = gradient slot:
+                   // accumulate Im <lambda| G |psi> over the tile
 };
 
 struct PassOp {
@@ -63,6 +67,8 @@ enum RegOpKind : int {
                     // slot of register j before / after (the lane's part is XORed in)
     RO_DENSE = 6,   // a = k, q[0..k) = local target bits, tmask = their mask, moff = matrix;
                     // off0[j] = LDS byte slot of register j, off1[j] = R(j), dep[c] = column c
+    RO_1QT = 7,     // a = register bit of the target, moff = coefficient table index (compiled passes only)
+    RO_GRAD = 8,    // a = register bit, q[0] = generator (0 X, 1 Y, 2 Z), moff = gradient slot
 };
 
 struct RegOp {
@@ -86,6 +92,10 @@ struct ProgramCache {
 };
 
 int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops);
+
+// A pass's ops (tile-local coordinates) as a register-resident program; rq receives the
+// register bits of the final layout.
+std::vector<RegOp> to_register_program(const std::vector<PassOp>& in, int L, int rq[4]);
 int jit_precompile(int n, const llsv_op* ops, uint64_t n_ops);
 int device_tile_bits(llsv_state* s);
 

@@ -548,6 +548,8 @@ uint32_t regs_with(uint32_t S, uint32_t m) {
     return r;
 }
 
+}  // namespace
+
 // Turns a pass's ops (tile-local coordinates) into the register-resident program: a layout
 // change is inserted before every one-qubit gate or CNOT whose target bit is not held in
 // registers; the new register set takes that bit, then the target bits of the next ops (so
@@ -558,7 +560,7 @@ std::vector<RegOp> to_register_program(const std::vector<PassOp>& in, int L, int
     constexpr uint32_t kStart = 0xF00u;  // local bits 8..11
     const uint32_t low = (1u << L) - 1;
     auto need_of = [](const PassOp& po) -> uint32_t {
-        if (po.kind == P_1Q) return 1u << po.a;
+        if (po.kind == P_1Q || po.kind == P_1QT || po.kind == P_GRAD) return 1u << po.a;
         if (po.kind == P_CNOT) return 1u << po.b;
         return 0u;
     };
@@ -635,6 +637,17 @@ std::vector<RegOp> to_register_program(const std::vector<PassOp>& in, int L, int
                 ro.kind = RO_SCALE;
                 ro.a = po.a;
                 break;
+            case P_1QT:
+                ro.kind = RO_1QT;
+                ro.a = rank(S, po.a);
+                ro.moff = po.moff;
+                break;
+            case P_GRAD:
+                ro.kind = RO_GRAD;
+                ro.a = rank(S, po.a);
+                ro.q[0] = po.b;
+                ro.moff = po.moff;
+                break;
             case P_DENSE: {
                 ro.kind = RO_DENSE;
                 ro.a = po.a;
@@ -662,8 +675,6 @@ std::vector<RegOp> to_register_program(const std::vector<PassOp>& in, int L, int
         if ((S >> bit) & 1u) rq[t++] = bit;
     return out;
 }
-
-}  // namespace
 
 namespace {
 

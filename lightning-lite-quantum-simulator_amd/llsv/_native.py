@@ -54,6 +54,7 @@ SIGNATURES = {
     "llsv_stream_sync": (_I, [_P]),
     "llsv_pc_workspace_bytes": (_U64, [_I, _I, _I]),
     "llsv_pc_energy": (_I, [_I, _I, _P, _I, _I, _P, _P, _I, _P, _P, _U64, _P]),
+    "llsv_pc_precompile": (_I, [_I, _P, _I, ctypes.POINTER(_I)]),
     "llsv_pc_adjoint": (_I, [_I, _I, _P, _I, _I, _P, _P, _I, _P, _P, _P, _U64, _P]),
     "llsv_traj_run": (_I, [_P, _I, _P, _I, _P, _P, _I, _U64, _P, _P, _P, _P]),
     "llsv_traj_plan_create": (_I, [_P, _I, _P, _I, _P, _P, _I, _I, ctypes.POINTER(_P)]),

@@ -66,6 +66,17 @@ class Circuit:
     def workspace_bytes(self, batch: int) -> int:
         return int(_native.lib().llsv_pc_workspace_bytes(self.n, batch, self.n_gates))
 
+    def precompile(self) -> tuple[int, int]:
+        """Compiles this circuit's tiled sweep kernels now, so that the first batched call does
+        not wait for them (llsv_pc_precompile).  Returns (forward passes, backward passes), both
+        0 when the sweeps run gate by gate (fewer than 20 qubits in the automatic compiler mode)."""
+        fwd = ctypes.c_int(0)
+        rc = _native.lib().llsv_pc_precompile(self.n, ctypes.cast(self.gates, ctypes.c_void_p), self.n_gates,
+                                              ctypes.byref(fwd))
+        if rc < 0:
+            _native.check(rc)
+        return int(fwd.value), int(rc) - int(fwd.value)
+
 
 class _Ordered:
     """The stream to launch on, ordered with torch's current stream.  With one HIP runtime in the

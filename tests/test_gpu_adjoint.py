@@ -96,6 +96,8 @@ def test_20q_hea_batch_vs_cpu_adjoint():
     terms = C.zz_x_hamiltonian(20, n)
     circ = LA.Circuit(n, gates, terms)
     th = np.random.default_rng(20).uniform(0, 2 * np.pi, (4, circ.n_params))
+    say("adjoint 20q batch 4: compile the tiled sweeps")
+    assert min(circ.precompile()) >= 1  # 20 qubits: tiled in the automatic mode
     say("adjoint 20q batch 4")
     e, g = LA.energies_and_grads(circ, cuda(th))
     ce, cg = A.adjoint(n, gates, th[3], terms)
@@ -104,6 +106,58 @@ def test_20q_hea_batch_vs_cpu_adjoint():
     # two entries by parameter shift on the CPU
     ps = A.param_shift(n, gates, th[3], terms, which=[0, 57])
     assert np.max(np.abs(ps - g[3, [0, 57]].cpu().numpy())) < 1e-10
+
+
+def _kinds_circuit(n, rng, layers=3):
+    from helpers import random_unitary
+
+    gates, p = [], 0
+    for layer in range(layers):
+        for q in range(n):
+            k = ("rx", "ry", "rz", "u")[(q + layer) % 4]
+            if k == "u":
+                gates.append(("u", q, random_unitary(rng, 2).reshape(4).tolist()))
+            else:
+                gates.append((k, q, p))
+                p += 1
+        for q in range(layer % 2, n - 1, 2):
+            gates.append(("cnot", q, q + 1) if layer % 2 == 0 else ("cz", q, q + 1))
+        gates.append(("cnot", n - 1, 0))
+        gates.append(("cz", 0, n - 1))
+    gates.append(("ry", n - 1, 0))  # a parameter used twice
+    return gates
+
+
+@pytest.mark.parametrize("n", [12, 13, 15])
+def test_tiled_sweeps_match_gate_by_gate(n):
+    # compiled 12-qubit register-tile sweeps (pass compiler forced on) vs one launch per gate
+    from llsv import _native
+    from llsv import adjoint as LA
+
+    rng = np.random.default_rng(100 + n)
+    gates = _kinds_circuit(n, rng)
+    terms = C.zz_x_hamiltonian(n, n) + [(0.4, 0b11, 0b01), (-0.7, 1 << (n - 1), 1 << (n - 1))]
+    circ = LA.Circuit(n, gates, terms)
+    th = rng.uniform(0, 2 * np.pi, (3, circ.n_params))
+    lib = _native.lib()
+    prev = lib.llsv_set_program_jit(0)
+    try:
+        say(f"adjoint gate by gate n={n}")
+        e0, g0 = LA.energies_and_grads(circ, cuda(th))
+        lib.llsv_set_program_jit(2)
+        npass = circ.precompile()
+        assert min(npass) >= 1
+        say(f"adjoint tiled n={n} passes={npass}")
+        e1, g1 = LA.energies_and_grads(circ, cuda(th))
+        e2 = LA.energies(circ, cuda(th))
+    finally:
+        lib.llsv_set_program_jit(prev)
+    e0, g0, e1, g1, e2 = (t.cpu().numpy() for t in (e0, g0, e1, g1, e2))
+    assert np.max(np.abs(e1 - e0)) < 1e-10 and np.max(np.abs(e2 - e0)) < 1e-10
+    assert np.max(np.abs(g1 - g0)) < 1e-10
+    ce, cg = A.adjoint(n, gates, th[2], terms)
+    assert abs(e1[2] - ce) < 1e-10
+    assert np.max(np.abs(g1[2] - cg)) < 1e-10
 
 
 def test_bad_arguments():
