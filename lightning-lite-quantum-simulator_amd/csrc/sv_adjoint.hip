@@ -424,6 +424,7 @@ std::vector<TiledPass> tiled_passes(int n, const std::vector<const llsv_pgate*>&
                 }
                 po.kind = P_1QT;
                 po.a = local_of[g.q0];
+                po.b = g.kind == LLSV_PG_RX ? 1 : g.kind == LLSV_PG_RY ? 2 : g.kind == LLSV_PG_RZ ? 3 : 0;
                 po.moff = coef_of[gi] + (backward ? ncoef : 0);
                 pops.push_back(po);
             }

@@ -135,13 +135,36 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
             case RO_1Q:
             case RO_1QT: {
                 const int K = 1 << op.a;
+                const int form = op.kind == RO_1QT ? op.q[0] : 0;
+                const size_t t8 = size_t(8) * size_t(op.moff);
                 if (op.kind == RO_1Q)
                     o.line("{ const double* m = M + %zu;", c);
                 else
-                    o.line("{ const double* m = M + %zu;", size_t(8) * size_t(op.moff));
-                for (int k = 0; k < na; ++k)
-                    for (int j = 0; j < 16; ++j)
-                        if (!(j & K)) o.line("r_pu(%s[%d], %s[%d], m);", arrs[k], j, arrs[k], j | K);
+                    o.line("{ const double* m = M + %zu;", t8);
+                if (form == 3) {
+                    if (op.tmask) {
+                        o.line("const bool hi = (lt & %uu) != 0u;", op.tmask);
+                        o.line("const double dr = hi ? m[6] : m[0], di = hi ? m[7] : m[1];");
+                        for (int k = 0; k < na; ++k)
+                            for (int j = 0; j < 16; ++j) o.line("r_sc(%s[%d], dr, di);", arrs[k], j);
+                    } else {
+                        for (int k = 0; k < na; ++k)
+                            for (int j = 0; j < 16; ++j)
+                                o.line("r_sc(%s[%d], m[%d], m[%d]);", arrs[k], j, (op.rmask >> j) & 1u ? 6 : 0,
+                                       (op.rmask >> j) & 1u ? 7 : 1);
+                    }
+                } else if (form == 1 || form == 2) {
+                    o.line("const double p = m[0], q = m[%d];", form == 1 ? 3 : 4);
+                    for (int k = 0; k < na; ++k)
+                        for (int j = 0; j < 16; ++j)
+                            if (!(j & K))
+                                o.line("%s(%s[%d], %s[%d], p, q);", form == 1 ? "r_rx" : "r_ry", arrs[k], j, arrs[k],
+                                       j | K);
+                } else {
+                    for (int k = 0; k < na; ++k)
+                        for (int j = 0; j < 16; ++j)
+                            if (!(j & K)) o.line("r_pu(%s[%d], %s[%d], m);", arrs[k], j, arrs[k], j | K);
+                }
                 o.line("}");
                 break;
             }
@@ -149,7 +172,12 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
                 // Im <lambda| G |psi> over this lane's pairs along register bit a (G = X, Y or Z on it)
                 const int K = 1 << op.a;
                 o.line("{ double g = 0.0;");
-                for (int j = 0; j < 16; ++j) {
+                if (op.q[0] == 2 && op.tmask) {
+                    // Z on a lane bit: the sign is the lane's
+                    for (int j = 0; j < 16; ++j) o.line("g += l[%d].x * a[%d].y - l[%d].y * a[%d].x;", j, j, j, j);
+                    o.line("if ((lt & %uu) != 0u) g = -g;", op.tmask);
+                }
+                for (int j = 0; j < 16 && !op.tmask; ++j) {
                     if (j & K) continue;
                     const int u = j | K;
                     if (op.q[0] == 2)

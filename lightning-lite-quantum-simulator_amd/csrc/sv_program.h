@@ -31,9 +31,11 @@ enum PassOpKind : int {
     P_MCZ = 4,     // negate where (l & lmask) == lmask, if (tile base & omask) == omask
     P_DENSE = 5,   // a = k, q[0..k) local bits, moff = offset (in v2d) of the matrix
     // batched parameterised circuits (register-resident passes only):
-    P_1QT = 6,     // a = local bit, moff = index of the 2x2 in the per-batch coefficient table
+    P_1QT = 6,     // a = local bit, moff = index of the 2x2 in the per-batch coefficient table,
+                   // b = its form: 0 general, 1 [[p, iq], [iq, p]] (RX), 2 [[p, -q], [q, p]] (RY),
+                   // 3 diagonal (RZ; needs no register residency)
     P_GRAD = 7,    // a = local bit, b = generator (0 X, 1 Y, 2 Z), moff = gradient slot:
-                   // accumulate Im <lambda| G |psi> over the tile
+                   // accumulate Im <lambda| G |psi> over the tile (Z needs no register residency)
 };
 
 struct PassOp {
@@ -67,8 +69,11 @@ enum RegOpKind : int {
                     // slot of register j before / after (the lane's part is XORed in)
     RO_DENSE = 6,   // a = k, q[0..k) = local target bits, tmask = their mask, moff = matrix;
                     // off0[j] = LDS byte slot of register j, off1[j] = R(j), dep[c] = column c
-    RO_1QT = 7,     // a = register bit of the target, moff = coefficient table index (compiled passes only)
-    RO_GRAD = 8,    // a = register bit, q[0] = generator (0 X, 1 Y, 2 Z), moff = gradient slot
+    RO_1QT = 7,     // a = register bit of the target, moff = coefficient table index, q[0] = form
+                    // (as P_1QT); form 3 (diagonal) as RO_DIAG: rmask, or tmask when the bit is
+                    // a lane bit (compiled passes only)
+    RO_GRAD = 8,    // a = register bit, q[0] = generator (0 X, 1 Y, 2 Z), moff = gradient slot;
+                    // Z on a lane bit: tmask = that bit
 };
 
 struct RegOp {

@@ -560,6 +560,8 @@ std::vector<RegOp> to_register_program(const std::vector<PassOp>& in, int L, int
     constexpr uint32_t kStart = 0xF00u;  // local bits 8..11
     const uint32_t low = (1u << L) - 1;
     auto need_of = [](const PassOp& po) -> uint32_t {
+        if (po.kind == P_1QT && po.b == 3) return 0u;   // diagonal: any residency
+        if (po.kind == P_GRAD && po.b == 2) return 0u;  // Z generator: any residency
         if (po.kind == P_1Q || po.kind == P_1QT || po.kind == P_GRAD) return 1u << po.a;
         if (po.kind == P_CNOT) return 1u << po.b;
         return 0u;
@@ -638,16 +640,19 @@ std::vector<RegOp> to_register_program(const std::vector<PassOp>& in, int L, int
                 ro.a = po.a;
                 break;
             case P_1QT:
-                ro.kind = RO_1QT;
-                ro.a = rank(S, po.a);
-                ro.moff = po.moff;
-                break;
-            case P_GRAD:
-                ro.kind = RO_GRAD;
-                ro.a = rank(S, po.a);
+            case P_GRAD: {
+                ro.kind = po.kind == P_1QT ? RO_1QT : RO_GRAD;
                 ro.q[0] = po.b;
                 ro.moff = po.moff;
+                const uint32_t bit = 1u << po.a;
+                if (S & bit) {
+                    ro.a = rank(S, po.a);
+                    ro.rmask = regs_with(S, bit);
+                } else {
+                    ro.tmask = bit;  // diagonal forms only (need_of)
+                }
                 break;
+            }
             case P_DENSE: {
                 ro.kind = RO_DENSE;
                 ro.a = po.a;

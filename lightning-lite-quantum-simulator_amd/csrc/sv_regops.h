@@ -12,6 +12,8 @@
 // r_ins0: inserts a zero bit at position pos.
 // r_pu: (x, y) <- (m00 x + m01 y, m10 x + m11 y), m row-major interleaved re/im.  Every output
 //   component is finished by the FMA that last reads the input component it replaces.
+// r_rx: (x, y) <- (p x + i q y, i q x + p y), the form of RX and its inverse (p, q real).
+// r_ry: (x, y) <- (p x - q y, q x + p y), the form of RY and its inverse (p, q real).
 // r_sc: v <- (dr + i di) v.
 // r_ng: flips the sign of both components when sgn = 0x80000000 (a sign-bit XOR, as -v);
 //   the components go through scalar locals (a bit cast of a vector element expression such
@@ -31,6 +33,20 @@
         x.y = __builtin_fma(m[0], x.y, px_im);                                                          \
         y.x = __builtin_fma(m[6], y.x, py_re);                                                          \
         y.y = __builtin_fma(m[6], y.y, py_im);                                                          \
+    }                                                                                                   \
+    __device__ __forceinline__ void r_rx(v2d& x, v2d& y, double p, double q) {                          \
+        const double xr = x.x, xi = x.y, yr = y.x, yi = y.y;                                            \
+        x.x = __builtin_fma(p, xr, -q * yi);                                                            \
+        x.y = __builtin_fma(p, xi, q * yr);                                                             \
+        y.x = __builtin_fma(p, yr, -q * xi);                                                            \
+        y.y = __builtin_fma(p, yi, q * xr);                                                             \
+    }                                                                                                   \
+    __device__ __forceinline__ void r_ry(v2d& x, v2d& y, double p, double q) {                          \
+        const double xr = x.x, xi = x.y, yr = y.x, yi = y.y;                                            \
+        x.x = __builtin_fma(p, xr, -q * yr);                                                            \
+        x.y = __builtin_fma(p, xi, -q * yi);                                                            \
+        y.x = __builtin_fma(p, yr, q * xr);                                                             \
+        y.y = __builtin_fma(p, yi, q * xi);                                                             \
     }                                                                                                   \
     __device__ __forceinline__ void r_sc(v2d& v, double dr, double di) {                                \
         const double pr = -di * v.y, pi = di * v.x;                                                     \
