@@ -311,6 +311,58 @@ int llsv_traj_plan_destroy(llsv_traj_plan* p);
  * fused): threads in {256, 512, 1024}, per_cu in {1, 2, 4}, threads * per_cu <= 2048. */
 int llsv_traj_default_config(int threads, int per_cu, int fused);
 
+/* ---- zero-noise extrapolation: folded circuits under sampled depolarizing noise ---- */
+/* The noise model of ll::NoisyBackend (src/Mitigation.h:196-303): every logical gate U runs as
+ * U (U^+ U)^k for scale 2k + 1, and after each application the touched qubits get depolarizing
+ * noise of total Pauli probability p, sampled per site: one channel for a 1-qubit gate, one
+ * 2-qubit channel for cnot and a 2-qubit mcz (Pauli index 1..15, low two bits on the first
+ * qubit), independent 1-qubit channels on each qubit of a wider mcz.  A Pauli Y is applied as
+ * Z then X, like the reference.  Trajectories run in one launch, each state in LDS. */
+#define LLSV_ZG_H 0
+#define LLSV_ZG_X 1
+#define LLSV_ZG_RX 2
+#define LLSV_ZG_RY 3
+#define LLSV_ZG_RZ 4
+#define LLSV_ZG_CNOT 5 /* q[0] = control, q[1] = target */
+#define LLSV_ZG_MCZ 6  /* q[0..nq) */
+#define LLSV_ZNE_MAX_QUBITS 12 /* a trajectory's state lives in one workgroup's LDS */
+
+typedef struct llsv_zgate {
+    int32_t kind; /* LLSV_ZG_* */
+    int32_t nq;   /* qubits in q (1 for one-qubit gates, 2 for cnot) */
+    int8_t q[16];
+    double theta; /* rx / ry / rz angle */
+} llsv_zgate;
+
+/* The sampler of one NoisyBackend: p, the random stream and its position.
+ * rng 0: the reference's stream -- std::mt19937_64 seeded with `seed`, u = (x >> 11) 2^-53 per
+ *        draw, one draw per channel and one more for the Pauli when u < p, consumed in order
+ *        (trajectory by trajectory, site by site), so the same seed and circuit give the same
+ *        Pauli insertions as ll::NoisyBackend; the draws are made on the host.
+ * rng 1: counter-based Philox4x32-10 on the device, counter (trajectory, site, slot); runs
+ *        continue the trajectory count, so successive runs draw fresh noise.
+ * Replaces ll::NoisyBackend's constructor (src/Mitigation.h:118-121): p outside [0, 1] (NaN
+ * included) is LLSV_ERR_ARG. */
+typedef struct llsv_noisy llsv_noisy;
+int llsv_noisy_create(double p, uint64_t seed, int rng, llsv_noisy** out);
+void llsv_noisy_destroy(llsv_noisy* z);
+/* NoisyBackend::reseed (src/Mitigation.h:123): restarts the stream (rng 1: the trajectory count). */
+int llsv_noisy_reseed(llsv_noisy* z, uint64_t seed);
+
+/* Draws one noise site of the reference stream (rng 0; NoisyBackend::noise,
+ * src/Mitigation.h:254-262), for a gate on nq qubits: *code gets two bits per channel slot
+ * (0 none, 1 X, 2 Y, 3 Z; slot i = i-th qubit of the gate).  For callers that apply gates one
+ * at a time. */
+int llsv_noisy_site(llsv_noisy* z, int nq, uint32_t* code);
+
+/* Runs n_traj trajectories of the logical circuit `gates` at noise scale `scale` (odd, 1..99)
+ * from |0...0> on n <= LLSV_ZNE_MAX_QUBITS qubits.  weights (host, 2^n doubles, or NULL): values[t]
+ * = sum_i weights[i] |psi_t[i]|^2 (host, n_traj doubles).  amps (host, n_traj * 2^n complex128
+ * interleaved, or NULL): the final states.  With p = 0 no draws are made.  Replaces the
+ * trajectory loop of ZNE::extrapolate (src/Mitigation.h:355-376) for one scale. */
+int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, int scale, int64_t n_traj,
+                   const double* weights, double* values, double* amps);
+
 /* ---- interop and measurement ------------------------------------------------ */
 
 /* Device address of the amplitudes and the state's stream (hipStream_t as void*). */

@@ -54,6 +54,11 @@ SIGNATURES = {
     "llsv_stream_sync": (_I, [_P]),
     "llsv_pc_workspace_bytes": (_U64, [_I, _I, _I]),
     "llsv_pc_energy": (_I, [_I, _I, _P, _I, _I, _P, _P, _I, _P, _P, _U64, _P]),
+    "llsv_noisy_create": (_I, [ctypes.c_double, _U64, _I, ctypes.POINTER(_P)]),
+    "llsv_noisy_destroy": (None, [_P]),
+    "llsv_noisy_reseed": (_I, [_P, _U64]),
+    "llsv_noisy_site": (_I, [_P, _I, ctypes.POINTER(ctypes.c_uint32)]),
+    "llsv_noisy_run": (_I, [_P, _I, _P, _I, _I, ctypes.c_int64, _P, _P, _P]),
     "llsv_pc_precompile": (_I, [_I, _P, _I, ctypes.POINTER(_I)]),
     "llsv_pc_adjoint": (_I, [_I, _I, _P, _I, _I, _P, _P, _I, _P, _P, _P, _U64, _P]),
     "llsv_traj_run": (_I, [_P, _I, _P, _I, _P, _P, _I, _U64, _P, _P, _P, _P]),
