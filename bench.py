@@ -315,6 +315,51 @@ def latency_leg(V3, n=12, n_ops=1000, seeds=10):
             "program_ms_per_circuit": round(t_prog * 1e3, 4), "program_ops_per_s": round(n_ops / t_prog, 1)}
 
 
+def zne_leg(n=12, layers=4, p=0.005, trajectories=8192):
+    """Zero-noise extrapolation of <sum_i Z_i> / n on a 12-qubit circuit (4 layers of RY + RZ on
+    every qubit and a CNOT ladder, 140 gates), depolarizing p = 0.005 per site, scales 1, 3, 5,
+    8192 trajectories per scale, every trajectory's state in LDS: with the reference's noise
+    stream (mt19937_64 draws on the host, the same Pauli insertions as the reference's
+    NoisyBackend for the same seed) and with device Philox draws."""
+    import quantum_sim_mitigation as QM
+
+    rng = np.random.Generator(np.random.PCG64(3))
+    gates = []
+    for _ in range(layers):
+        for q in range(n):
+            gates.append(("ry", q, float(rng.uniform(0, 2 * np.pi))))
+            gates.append(("rz", q, float(rng.uniform(0, 2 * np.pi))))
+        for q in range(n - 1):
+            gates.append(("cnot", q, q + 1))
+    w = QM.Diagonal.from_function(n, lambda i: sum(1.0 - 2.0 * ((i >> q) & 1) for q in range(n)) / n)
+
+    def circ(b):
+        for g in gates:
+            if g[0] == "cnot":
+                b.cnot(g[1], g[2])
+            else:
+                getattr(b, g[0])(g[1], g[2])
+
+    out = {"workload": f"{n}-qubit circuit, {len(gates)} gates ({layers} layers RY+RZ + CNOT ladder), "
+                       f"depolarizing p = {p}, scales 1/3/5, {trajectories} trajectories per scale, complex128"}
+    apps = len(gates) * (1 + 3 + 5) * trajectories
+    for mode in ("reference", "philox"):
+        nb = QM.NoisyBackend(QM.ReferenceBackend(n), p, 7, rng=mode)
+        QM.ZNE(256).extrapolate(nb, circ, w, [1, 3, 5])  # warm-up (buffers, module load)
+        t0 = time.perf_counter()
+        r = QM.ZNE(trajectories).extrapolate(nb, circ, w, [1, 3, 5])
+        dt = time.perf_counter() - t0
+        log(f"zne ({mode}): {3 * trajectories} trajectories in {dt:.3f} s, E0 = {r.value:.6f} +- {r.std_error:.1e} "
+            f"(unmitigated {r.values[0]:.6f})")
+        out[mode] = {"s_per_extrapolation": round(dt, 4), "trajectories_per_s": round(3 * trajectories / dt, 1),
+                     "gate_applications_per_s": round(apps / dt, 1), "value": r.value, "std_error": r.std_error,
+                     "unmitigated": r.values[0]}
+    # the noiseless value (p = 0: one trajectory)
+    ideal = QM.ZNE(1).extrapolate(QM.NoisyBackend(QM.ReferenceBackend(n), 0.0, 1), circ, w, [1, 3]).value
+    out["ideal"] = ideal
+    return out
+
+
 def program_leg(V3, n=28, layers=20):
     """Only the program path on the 28-qubit layered circuit (for counter runs)."""
     from llsv import circuits as C, program as P
@@ -395,8 +440,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the layered, adjoint, trajectory and 32-qubit legs")
-    ap.add_argument("--leg", choices=["latency12", "layered28", "program28", "adjoint20", "trajectories14", "q32"],
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the latency, layered, adjoint, trajectory, ZNE and 32-qubit legs")
+    ap.add_argument("--leg", choices=["latency12", "layered28", "program28", "adjoint20", "trajectories14", "zne12",
+                                      "q32"],
                     help="run only this leg (for profiling) and print its JSON")
     ap.add_argument("--probe", action="store_true",
                     help="launch each kernel a few times and exit (for rocprofv3 counter runs)")
@@ -416,7 +463,7 @@ def main():
 
         fn = {"latency12": lambda: latency_leg(V3), "layered28": lambda: layered_leg(V3),
               "program28": lambda: program_leg(V3),
-              "adjoint20": adjoint_leg, "trajectories14": trajectory_leg,
+              "adjoint20": adjoint_leg, "trajectories14": trajectory_leg, "zne12": zne_leg,
               "q32": lambda: q32_leg(V3)}[a.leg]
         print(json.dumps({a.leg: fn()}), flush=True)
         return
@@ -577,6 +624,7 @@ def main():
         line["layered28"] = layered_leg(V3)
         line["adjoint20"] = adjoint_leg()
         line["trajectories14"] = trajectory_leg()
+        line["zne12"] = zne_leg()
         line["q32"] = q32_leg(V3)
     if not a.no_cpu_baseline:
         log("cpu baseline (subprocess)")
