@@ -35,6 +35,7 @@ extern "C" {
 #define LLSV_ERR_NOMEM (-2)  /* device or pinned-host allocation failed */
 #define LLSV_ERR_DEVICE (-3) /* HIP runtime error */
 #define LLSV_ERR_HANDLE (-4) /* null handle */
+#define LLSV_ERR_NUMERIC (-5) /* numerical breakdown (the reference's std::runtime_error, e.g. a vanishing norm) */
 
 #define LLSV_MAX_QUBITS 34   /* 2^34 x 16 B = 256 GiB, the most one 288 GB HBM3E device holds */
 #define LLSV_MAX_DENSE_K 5   /* largest fused block: 32 x 32 */
@@ -362,6 +363,38 @@ int llsv_noisy_site(llsv_noisy* z, int nq, uint32_t* code);
  * trajectory loop of ZNE::extrapolate (src/Mitigation.h:355-376) for one scale. */
 int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, int scale, int64_t n_traj,
                    const double* weights, double* values, double* amps);
+
+/* ---- dense Lindblad trajectories (<= 10 qubits) ------------------------------------ */
+/* The trajectory solver of src/TrajectorySolver.h: many trajectories of a dim = 2^n state
+ * (n <= 10) under a dense propagator, stepped together -- each step's propagation of all
+ * trajectories is one complex GEMM on the FP64 matrix cores.  Matrices are dim x dim complex128,
+ * row-major, interleaved (re, im). */
+
+/* exp(A) by scaling and squaring with an 18-term Taylor series, ||A / 2^s||_inf <= 1/2
+ * (traj_detail::expm, src/TrajectorySolver.h:97-111), the products on the GPU.  Host in/out. */
+int llsv_expm(int dim, const double* a, double* out);
+
+/* A plan (TrajectorySolver::Plan, src/TrajectorySolver.h:291-301) on the device.
+ * kind 0: jumps (STANDARD / PROJECTOR; src/TrajectorySolver.h:379-428): nodes = 1 propagator
+ *         exp(-i H_eff h), or 2 steps + 1 matrices -i (H(t_j) + H_nh) at t_j = j h / 2 (RK4);
+ *         ops = the n_ops jump operators.
+ * kind 1: diffusive (ANALOG; src/TrajectorySolver.h:432-460): nodes = 1 propagator exp(-i H h),
+ *         or `steps` matrices -i H(t_j) at t_j = j h (Euler drift); ops = the C_k.
+ * n_ops <= 16. */
+typedef struct llsv_lplan llsv_lplan;
+int llsv_lplan_create(int dim, int kind, int64_t steps, double h, int n_nodes, const double* nodes, int n_ops,
+                      const double* ops, llsv_lplan** out);
+void llsv_lplan_destroy(llsv_lplan* p);
+
+/* Runs trajectories first_traj .. first_traj + n_traj - 1 of an ensemble from psi0 (dim complex,
+ * normalised): states (host, n_traj x dim complex128) gets the final normalised states.
+ * rng 0: trajectory i draws from std::mt19937_64(splitmix64(seed ^ splitmix64(i + 1))) with
+ * std::uniform_real_distribution<double>(0, 1) (jumps) or std::normal_distribution<double>(0,
+ * sqrt(h)) (diffusive), like TrajectorySolver::make_rng (src/TrajectorySolver.h:311-313), so an
+ * ensemble reproduces the reference's draws; rng 1: Philox4x32-10 on the device.  A vanishing or
+ * diverging norm is LLSV_ERR_NUMERIC with the reference's message. */
+int llsv_lplan_run(llsv_lplan* p, const double* psi0, int64_t n_traj, int64_t first_traj, uint64_t seed, int rng,
+                   double* states);
 
 /* ---- interop and measurement ------------------------------------------------ */
 

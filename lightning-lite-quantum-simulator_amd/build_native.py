@@ -19,7 +19,7 @@ INC = os.path.join(os.path.dirname(PKG), "include")
 OBJ = os.path.join(PKG, "_build")
 ARCH = os.environ.get("LLSV_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["sv_gates.hip", "sv_reduce.hip", "sv_api.hip", "sv_program.hip", "sv_dense.hip", "sv_adjoint.hip", "sv_traj.hip", "sv_jit.hip", "sv_readout.hip", "sv_zne.hip"]
+HIP_SOURCES = ["sv_gates.hip", "sv_reduce.hip", "sv_api.hip", "sv_program.hip", "sv_dense.hip", "sv_adjoint.hip", "sv_traj.hip", "sv_jit.hip", "sv_readout.hip", "sv_zne.hip", "sv_lindblad.hip"]
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC", "-Wall", "-Wno-unused-result",
     # packed-FP32 instructions stay off: every kernel here is FP64 anyway.

@@ -1,0 +1,696 @@
+// sv_lindblad.hip -- batched Lindblad quantum trajectories with dense operators (<= 10 qubits):
+// the propagation prop . psi of every trajectory in a step is one complex GEMM on the FP64 matrix
+// cores, Psi_next^T = Psi^T prop^T (trajectories as rows).
+//
+// k_zgemm: C = A B for complex128 row-major matrices, M % 64 == 0, N % 64 == 0, K % 16 == 0 (callers
+// pad with zeros).  A 256-thread workgroup computes a 64 x 64 block of C; each wavefront a 32 x 32
+// quarter as 2 x 2 blocks of v_mfma_f64_16x16x4f64, complex through the real embedding
+//     Cr += Ar Br - Ai Bi,   Ci += Ar Bi + Ai Br          (four MFMAs per block and K step).
+// A and B tiles (64 x 16 and 16 x 64) are staged in LDS as separate real and imaginary planes,
+// K-major, so every MFMA operand is one conflict-free 8-byte LDS read per lane.
+//
+// The jump algorithm (src/TrajectorySolver.h:379-428) per step, for all trajectories at once:
+//   k_zgemm       tmp = prop psi                         (or the RK4 stages for a time-dependent H)
+//   k_jump_decide one wavefront per trajectory: survive = ||tmp||^2; jump iff u >= survive (u the
+//                 trajectory's next uniform); otherwise psi = tmp / sqrt(survive)
+//   k_jump_apply  one workgroup per jumping trajectory: w_k = ||C_k tmp||^2 for every channel,
+//                 channel by cumulative weight (a second uniform), psi = C_k tmp / sqrt(w_k)
+// The uniforms are the reference's: per trajectory std::mt19937_64 seeded with
+// splitmix64(seed ^ splitmix64(i + 1)) and std::uniform_real_distribution<double>(0, 1), drawn on
+// the host (at most two per step) -- or counter-based Philox draws on the device.
+// The diffusive unraveling (src/TrajectorySolver.h:432-460) stacks every C_k and C_k^+ C_k into
+// one GEMM per step and updates each trajectory in one wavefront (k_diffuse).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <thread>
+#include <vector>
+
+#include "../../include/llsv.h"
+#include "sv_common.h"
+
+namespace llsv {
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kBM = 64, kBN = 64, kBK = 16;
+
+__global__ __launch_bounds__(256) void k_zgemm(const v2d* __restrict__ A, int lda, const v2d* __restrict__ B,
+                                               int ldb, v2d* __restrict__ C, int ldc, int K) {
+    __shared__ double as_r[kBK][kBM], as_i[kBK][kBM], bs_r[kBK][kBN], bs_i[kBK][kBN];
+    const int tid = int(threadIdx.x), lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int m0 = int(blockIdx.y) * kBM, n0 = int(blockIdx.x) * kBN;
+    d4 cr[2][2], ci[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+            cr[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+            ci[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+        }
+    // tile loads: A row ar = tid / 4, columns 4 (tid % 4) .. +3; B row br = tid / 16, columns 4 (tid % 16) .. +3
+    const int ar = tid >> 2, ac = (tid & 3) * 4, br = tid >> 4, bc = (tid & 15) * 4;
+    for (int k0 = 0; k0 < K; k0 += kBK) {
+        const v2d* pa = A + size_t(m0 + ar) * lda + k0 + ac;
+        const v2d* pb = B + size_t(k0 + br) * ldb + n0 + bc;
+        v2d va[4], vb[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            va[j] = pa[j];
+            vb[j] = pb[j];
+        }
+        __syncthreads();  // the previous tile is no longer read
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            as_r[ac + j][ar] = va[j].x;
+            as_i[ac + j][ar] = va[j].y;
+            bs_r[br][bc + j] = vb[j].x;
+            bs_i[br][bc + j] = vb[j].y;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < kBK / 4; ++ks) {
+            const int kk = ks * 4 + (lane >> 4);
+            double a_r[2], a_i[2], a_n[2], b_r[2], b_i[2];
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb) {
+                a_r[mb] = as_r[kk][wm * 32 + mb * 16 + (lane & 15)];
+                a_i[mb] = as_i[kk][wm * 32 + mb * 16 + (lane & 15)];
+                a_n[mb] = -a_i[mb];
+            }
+#pragma unroll
+            for (int nb = 0; nb < 2; ++nb) {
+                b_r[nb] = bs_r[kk][wn * 32 + nb * 16 + (lane & 15)];
+                b_i[nb] = bs_i[kk][wn * 32 + nb * 16 + (lane & 15)];
+            }
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+                for (int nb = 0; nb < 2; ++nb) {
+                    cr[mb][nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[mb], b_r[nb], cr[mb][nb], 0, 0, 0);
+                    ci[mb][nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[mb], b_i[nb], ci[mb][nb], 0, 0, 0);
+                }
+#pragma unroll
+            for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+                for (int nb = 0; nb < 2; ++nb) {
+                    cr[mb][nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_n[mb], b_i[nb], cr[mb][nb], 0, 0, 0);
+                    ci[mb][nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i[mb], b_r[nb], ci[mb][nb], 0, 0, 0);
+                }
+        }
+    }
+    // D[i][j] of a 16 x 16 block: lane j + 16 (i mod 4), element i / 4
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = m0 + wm * 32 + mb * 16 + 4 * v + (lane >> 4);
+                const int col = n0 + wn * 32 + nb * 16 + (lane & 15);
+                v2d y;
+                y.x = cr[mb][nb][v];
+                y.y = ci[mb][nb][v];
+                C[size_t(row) * ldc + col] = y;
+            }
+}
+
+// ---- small elementwise kernels -------------------------------------------------------------
+
+// X[i] = X[i] / d (both components), n elements.
+__global__ void k_div(v2d* __restrict__ X, uint64_t n, double d) {
+    for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += uint64_t(gridDim.x) * 256) {
+        v2d v = X[i];
+        v.x = v.x / d;
+        v.y = v.y / d;
+        X[i] = v;
+    }
+}
+
+// Y[i] += X[i].
+__global__ void k_add(v2d* __restrict__ Y, const v2d* __restrict__ X, uint64_t n) {
+    for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += uint64_t(gridDim.x) * 256) Y[i] = Y[i] + X[i];
+}
+
+// Y = X + c Z (real c).
+__global__ void k_axpy(v2d* __restrict__ Y, const v2d* __restrict__ X, const v2d* __restrict__ Z, double c, uint64_t n) {
+    for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += uint64_t(gridDim.x) * 256) {
+        const v2d x = X[i], z = Z[i];
+        v2d y;
+        y.x = x.x + c * z.x;
+        y.y = x.y + c * z.y;
+        Y[i] = y;
+    }
+}
+
+// RK4 combination: T = P + (h / 6) (K1 + 2 K2 + 2 K3 + K4), in the reference's order of operations.
+__global__ void k_rk4(v2d* __restrict__ T, const v2d* __restrict__ P, const v2d* __restrict__ K1,
+                      const v2d* __restrict__ K2, const v2d* __restrict__ K3, const v2d* __restrict__ K4, double h6,
+                      uint64_t n) {
+    for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += uint64_t(gridDim.x) * 256) {
+        const v2d s = ((K1[i] + 2.0 * K2[i]) + 2.0 * K3[i]) + K4[i];
+        T[i] = P[i] + h6 * s;
+    }
+}
+
+__global__ void k_identity(v2d* __restrict__ X, int D) {
+    const uint64_t n = uint64_t(D) * D;
+    for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += uint64_t(gridDim.x) * 256)
+        X[i] = v2d{(i / D == i % D) ? 1.0 : 0.0, 0.0};
+}
+
+// ---- random numbers ---------------------------------------------------------------------------
+
+__device__ __host__ inline void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = uint64_t(0xD2511F53u) * c[0];
+        const uint64_t p1 = uint64_t(0xCD9E8D57u) * c[2];
+        const uint32_t n0 = uint32_t(p1 >> 32) ^ c[1] ^ k0;
+        const uint32_t n2 = uint32_t(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = uint32_t(p1);
+        c[3] = uint32_t(p0);
+        c[0] = n0;
+        c[2] = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// The c-th uniform of trajectory t: a host-drawn table (stride per trajectory) or Philox.
+struct Draws {
+    const double* table;  // null: Philox
+    uint64_t stride;
+    uint64_t seed;
+    uint64_t first;  // index of trajectory 0 of this launch in the ensemble
+    __device__ double u(uint64_t t, uint64_t c) const {
+        if (table) return table[t * stride + c];
+        uint32_t x[4] = {uint32_t(first + t), uint32_t((first + t) >> 32), uint32_t(c), uint32_t(c >> 32) ^ 0x4C494E44u};
+        philox10(x, uint32_t(seed), uint32_t(seed >> 32));
+        const uint64_t v = uint64_t(x[0]) | (uint64_t(x[1]) << 32);
+        return double(v >> 11) * (1.0 / 9007199254740992.0);
+    }
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ---- jump steps -----------------------------------------------------------------------------
+
+// One wavefront per trajectory.  state[t]: 0 running, 1 jump pending (set here), 2 norm vanished.
+__global__ __launch_bounds__(256) void k_jump_decide(const v2d* __restrict__ tmp, v2d* __restrict__ psi, int D,
+                                                     uint64_t T, int nch, Draws dr, uint64_t* __restrict__ cursor,
+                                                     double* __restrict__ survive_out, int* __restrict__ flag) {
+    const uint64_t t = uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int lane = int(threadIdx.x & 63);
+    if (t >= T) return;
+    const v2d* x = tmp + t * D;
+    double s = 0.0;
+    for (int i = lane; i < D; i += 64) s += x[i].x * x[i].x + x[i].y * x[i].y;
+    s = wave_sum(s);
+    bool jump = false;
+    if (!(s > 0.0) || !(s <= 1.7976931348623157e308)) {
+        if (lane == 0) atomicOr(flag, 1);
+        s = 1.0;
+    } else if (nch > 0) {
+        const uint64_t c = cursor[t];
+        jump = dr.u(t, c) >= s;
+        if (lane == 0) cursor[t] = c + 1;
+    }
+    if (lane == 0) survive_out[t] = jump ? -s : s;  // negative: jump pending
+    if (jump) return;
+    const double r = sqrt(s);
+    v2d* y = psi + t * D;
+    for (int i = lane; i < D; i += 64) {
+        v2d v = x[i];
+        v.x = v.x / r;
+        v.y = v.y / r;
+        y[i] = v;
+    }
+}
+
+// One workgroup per trajectory (those without a pending jump return at once).  ops: nch
+// D x D operators, row-major.
+__global__ __launch_bounds__(256) void k_jump_apply(const v2d* __restrict__ tmp, v2d* __restrict__ psi, int D,
+                                                    uint64_t T, int nch, const v2d* __restrict__ ops, Draws dr,
+                                                    uint64_t* __restrict__ cursor, const double* __restrict__ survive) {
+    extern __shared__ v2d xs[];  // tmp of this trajectory (D)
+    __shared__ double red[4];
+    __shared__ double wk[64];
+    const uint64_t t = blockIdx.x;
+    if (t >= T || !(survive[t] < 0.0)) return;
+    const int tid = int(threadIdx.x), lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < D; i += 256) xs[i] = tmp[t * D + i];
+    __syncthreads();
+    // ||C_k tmp||^2 for every channel: one row per wavefront at a time, lanes along the row
+    for (int k = 0; k < nch; ++k) {
+        const v2d* M = ops + size_t(k) * D * D;
+        double acc = 0.0;
+        for (int r = wave; r < D; r += 4) {
+            double yr = 0.0, yi = 0.0;
+            for (int j = lane; j < D; j += 64) {
+                const v2d m = M[size_t(r) * D + j], x = xs[j];
+                yr += m.x * x.x - m.y * x.y;
+                yi += m.x * x.y + m.y * x.x;
+            }
+            yr = wave_sum(yr);
+            yi = wave_sum(yi);
+            acc += yr * yr + yi * yi;
+        }
+        if (lane == 0) red[wave] = acc;
+        __syncthreads();
+        if (tid == 0) wk[k] = (red[0] + red[1]) + (red[2] + red[3]);
+        __syncthreads();
+    }
+    // channel choice (src/TrajectorySolver.h:395-417), same in every thread
+    double total = 0.0;
+    for (int k = 0; k < nch; ++k) total += wk[k];
+    if (!(total > 0.0)) {
+        // no channel can fire: the no-jump branch
+        const double r = sqrt(-survive[t]);
+        for (int i = tid; i < D; i += 256) {
+            v2d v = xs[i];
+            v.x = v.x / r;
+            v.y = v.y / r;
+            psi[t * D + i] = v;
+        }
+        return;
+    }
+    const uint64_t c = cursor[t];
+    const double rr = dr.u(t, c) * total;
+    int pick = nch;
+    double acc = 0.0;
+    for (int k = 0; k < nch; ++k) {
+        acc += wk[k];
+        if (rr < acc && wk[k] > 0.0) {
+            pick = k;
+            break;
+        }
+    }
+    if (pick == nch)
+        for (int k = nch; k-- > 0;)
+            if (wk[k] > 0.0) {
+                pick = k;
+                break;
+            }
+    __syncthreads();
+    if (tid == 0) cursor[t] = c + 1;
+    const double nrm = sqrt(wk[pick]);
+    const v2d* M = ops + size_t(pick) * D * D;
+    for (int r = wave; r < D; r += 4) {
+        double yr = 0.0, yi = 0.0;
+        for (int j = lane; j < D; j += 64) {
+            const v2d m = M[size_t(r) * D + j], x = xs[j];
+            yr += m.x * x.x - m.y * x.y;
+            yi += m.x * x.y + m.y * x.x;
+        }
+        yr = wave_sum(yr);
+        yi = wave_sum(yi);
+        if (lane == 0) psi[t * D + r] = v2d{yr / nrm, yi / nrm};
+    }
+}
+
+// ---- diffusive steps ------------------------------------------------------------------------
+
+// One wavefront per trajectory.  cp = [C_1 psi | .. | C_n psi | D_1 psi | .. | D_n psi] (row t,
+// 2 n D entries, D_k = C_k^+ C_k); drift (D entries, or null: zero); psi updated in place.
+// normals: n per step per trajectory (host table; Philox mode draws Box-Muller pairs).
+__global__ __launch_bounds__(256) void k_diffuse(v2d* __restrict__ psi, const v2d* __restrict__ cp,
+                                                 const v2d* __restrict__ drift_in, int D, uint64_t T, int nch,
+                                                 double h, double sq_h, const double* __restrict__ normals,
+                                                 uint64_t nstride, uint64_t step, Draws dr, int* __restrict__ flag) {
+    const uint64_t t = uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int lane = int(threadIdx.x & 63);
+    if (t >= T) return;
+    v2d* p = psi + t * D;
+    const v2d* c = cp + t * size_t(2 * nch) * D;
+    constexpr int kMaxCh = 16;
+    double x[kMaxCh], g[kMaxCh];
+    for (int k = 0; k < nch; ++k) {
+        // x_k = 2 Re <psi | C_k psi>
+        double s = 0.0;
+        for (int i = lane; i < D; i += 64) {
+            const v2d a = p[i], b = c[size_t(k) * D + i];
+            s += a.x * b.x + a.y * b.y;
+        }
+        x[k] = 2.0 * wave_sum(s);
+        if (normals) {
+            g[k] = normals[t * nstride + step * nch + k];
+        } else {  // Box-Muller on two Philox uniforms, scaled by sqrt(h)
+            const double u1 = dr.u(t, 2 * (step * nch + k)), u2 = dr.u(t, 2 * (step * nch + k) + 1);
+            g[k] = sq_h * sqrt(-2.0 * log(1.0 - u1)) * cos(6.283185307179586 * u2);
+        }
+    }
+    double nn = 0.0;
+    for (int i = lane; i < D; i += 64) {
+        const v2d a = p[i];
+        v2d drift = drift_in ? drift_in[t * D + i] : v2d{0.0, 0.0};
+        v2d noise = v2d{0.0, 0.0};
+        for (int k = 0; k < nch; ++k) {
+            const v2d cpk = c[size_t(k) * D + i], t2 = c[size_t(nch + k) * D + i];
+            // drift += -0.5 (t2 - x cp + 0.25 x^2 psi); noise += (cp - 0.5 x psi) g
+            const v2d inner = (t2 - x[k] * cpk) + (0.25 * x[k] * x[k]) * a;
+            drift = drift + (-0.5) * inner;
+            noise = noise + (cpk - (0.5 * x[k]) * a) * g[k];
+        }
+        const v2d v = a + (h * drift + noise);  // psi += h drift + noise
+        p[i] = v;
+        nn += v.x * v.x + v.y * v.y;
+    }
+    nn = sqrt(wave_sum(nn));
+    if (!(nn > 0.0) || !(nn <= 1.7976931348623157e308)) {
+        if (lane == 0) atomicOr(flag, 2);
+        return;
+    }
+    for (int i = lane; i < D; i += 64) {
+        v2d v = p[i];
+        v.x = v.x / nn;
+        v.y = v.y / nn;
+        p[i] = v;
+    }
+}
+
+// ---- host ----------------------------------------------------------------------------------
+
+int pad_dim(int dim) { return dim < kBN ? kBN : dim; }
+
+hipError_t gemm(hipStream_t st, const v2d* A, int lda, const v2d* B, int ldb, v2d* C, int ldc, int M, int N, int K) {
+    hipLaunchKernelGGL(k_zgemm, dim3(unsigned(N / kBN), unsigned(M / kBM)), dim3(256), 0, st, A, lda, B, ldb, C, ldc, K);
+    return hipGetLastError();
+}
+
+unsigned grid_of(uint64_t n) { return unsigned(std::min<uint64_t>((n + 255) / 256, 65536)); }
+
+uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+
+// Host upload of a dim x dim complex matrix (row-major) into a D x D device block, transposed
+// when `transpose` (the GEMM's B operand is prop^T), zero padding.
+std::vector<double> padded(const double* m, int dim, int D, bool transpose) {
+    std::vector<double> out(size_t(2) * D * D, 0.0);
+    for (int i = 0; i < dim; ++i)
+        for (int j = 0; j < dim; ++j) {
+            const size_t src = 2 * (size_t(i) * dim + j);
+            const size_t dst = transpose ? 2 * (size_t(j) * D + i) : 2 * (size_t(i) * D + j);
+            out[dst] = m[src];
+            out[dst + 1] = m[src + 1];
+        }
+    return out;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t bytes) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        return hipMalloc(&p, bytes ? bytes : 16);
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+}  // namespace
+}  // namespace llsv
+
+struct llsv_lplan {
+    int dim = 0, D = 0, kind = 0, nch = 0, n_nodes = 0;
+    int64_t steps = 0;
+    double h = 0.0;
+    llsv::DevBuf nodes;  // propagator or RK4 / drift nodes, transposed and padded (D x D each)
+    llsv::DevBuf ops;    // jump operators (padded, row-major), or for ANALOG the stacked
+                         // [C_1^T .. C_n^T | D_1^T .. D_n^T] (D x 2nD)
+};
+
+extern "C" {
+
+int llsv_expm(int dim, const double* a, double* out) {
+    using namespace llsv;
+    if (dim < 1 || dim > 4096 || !a || !out) return fail(LLSV_ERR_ARG, "llsv_expm: bad arguments");
+    // exp(A) by scaling and squaring: ||A / 2^s||_inf <= 1/2, 18-term Taylor series
+    // (src/TrajectorySolver.h:97-111)
+    double nrm = 0.0;
+    for (int i = 0; i < dim; ++i) {
+        double r = 0.0;
+        for (int j = 0; j < dim; ++j) r += std::hypot(a[2 * (size_t(i) * dim + j)], a[2 * (size_t(i) * dim + j) + 1]);
+        nrm = std::max(nrm, r);
+    }
+    int s = 0;
+    if (nrm > 0.5) s = std::max(0, int(std::ceil(std::log2(nrm / 0.5))));
+    const int D = ((std::max(dim, kBN) + kBN - 1) / kBN) * kBN;
+    const uint64_t nn = uint64_t(D) * D;
+    DevBuf b, term, res, tmp;
+#define XK(expr)                                                                                      \
+    do {                                                                                              \
+        const hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_expm: %s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+    XK(b.alloc(nn * 16));
+    XK(term.alloc(nn * 16));
+    XK(res.alloc(nn * 16));
+    XK(tmp.alloc(nn * 16));
+    const std::vector<double> hb = padded(a, dim, D, false);
+    XK(hipMemcpy(b.p, hb.data(), nn * 16, hipMemcpyHostToDevice));
+    const hipStream_t st = nullptr;
+    hipLaunchKernelGGL(k_div, dim3(grid_of(nn)), dim3(256), 0, st, b.as<v2d>(), nn, std::pow(2.0, s));
+    hipLaunchKernelGGL(k_identity, dim3(grid_of(nn)), dim3(256), 0, st, term.as<v2d>(), D);
+    hipLaunchKernelGGL(k_identity, dim3(grid_of(nn)), dim3(256), 0, st, res.as<v2d>(), D);
+    XK(hipGetLastError());
+    for (int k = 1; k <= 18; ++k) {
+        XK(gemm(st, term.as<v2d>(), D, b.as<v2d>(), D, tmp.as<v2d>(), D, D, D, D));
+        hipLaunchKernelGGL(k_div, dim3(grid_of(nn)), dim3(256), 0, st, tmp.as<v2d>(), nn, double(k));
+        std::swap(term.p, tmp.p);
+        hipLaunchKernelGGL(k_add, dim3(grid_of(nn)), dim3(256), 0, st, res.as<v2d>(), term.as<const v2d>(), nn);
+        XK(hipGetLastError());
+    }
+    for (int i = 0; i < s; ++i) {
+        XK(gemm(st, res.as<v2d>(), D, res.as<v2d>(), D, tmp.as<v2d>(), D, D, D, D));
+        std::swap(res.p, tmp.p);
+    }
+    std::vector<double> hr(2 * nn);
+    XK(hipMemcpy(hr.data(), res.p, nn * 16, hipMemcpyDeviceToHost));
+#undef XK
+    for (int i = 0; i < dim; ++i)
+        std::memcpy(out + 2 * size_t(i) * dim, hr.data() + 2 * size_t(i) * D, sizeof(double) * 2 * dim);
+    return LLSV_OK;
+}
+
+int llsv_lplan_create(int dim, int kind, int64_t steps, double h, int n_nodes, const double* nodes, int n_ops,
+                      const double* ops, llsv_lplan** out) {
+    using namespace llsv;
+    if (!out) return fail(LLSV_ERR_ARG, "llsv_lplan_create: out is null");
+    *out = nullptr;
+    if (dim < 1 || dim > 1024 || (dim & (dim - 1))) return fail(LLSV_ERR_ARG, "llsv_lplan_create: dim must be 2^n, n <= 10");
+    if (kind < 0 || kind > 1) return fail(LLSV_ERR_ARG, "llsv_lplan_create: kind must be 0 (jumps) or 1 (diffusive)");
+    if (steps < 1 || !(h > 0.0)) return fail(LLSV_ERR_ARG, "llsv_lplan_create: need steps >= 1 and h > 0");
+    if (n_ops < 0 || n_ops > 16 || (n_ops && !ops)) return fail(LLSV_ERR_ARG, "llsv_lplan_create: 0..16 channels");
+    const int64_t want = n_nodes == 1 ? 1 : (kind == 0 ? 2 * steps + 1 : steps);
+    if (!nodes || (n_nodes != 1 && n_nodes != want))
+        return fail(LLSV_ERR_ARG, "llsv_lplan_create: %d nodes given, 1 or %lld expected", n_nodes, (long long)want);
+    auto p = std::make_unique<llsv_lplan>();
+    p->dim = dim;
+    p->D = pad_dim(dim);
+    p->kind = kind;
+    p->nch = n_ops;
+    p->n_nodes = n_nodes;
+    p->steps = steps;
+    p->h = h;
+    const int D = p->D;
+    const size_t blk = size_t(D) * D * 16;
+#define PK(expr)                                                                                      \
+    do {                                                                                              \
+        const hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_lplan_create: %s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+    PK(p->nodes.alloc(blk * size_t(n_nodes)));
+    for (int k = 0; k < n_nodes; ++k) {
+        const std::vector<double> m = padded(nodes + 2 * size_t(k) * dim * dim, dim, D, true);
+        PK(hipMemcpy(p->nodes.as<char>() + blk * k, m.data(), blk, hipMemcpyHostToDevice));
+    }
+    if (kind == 0) {
+        PK(p->ops.alloc(blk * size_t(std::max(1, n_ops))));
+        for (int k = 0; k < n_ops; ++k) {
+            const std::vector<double> m = padded(ops + 2 * size_t(k) * dim * dim, dim, D, false);
+            PK(hipMemcpy(p->ops.as<char>() + blk * k, m.data(), blk, hipMemcpyHostToDevice));
+        }
+    } else {
+        // B = [C_1^T | .. | C_n^T | D_1^T | .. | D_n^T], D x 2nD, D_k = C_k^+ C_k (computed here)
+        const int W = 2 * n_ops * D;
+        std::vector<double> B(size_t(2) * D * std::max(W, 1), 0.0);
+        for (int k = 0; k < n_ops; ++k) {
+            const double* c = ops + 2 * size_t(k) * dim * dim;
+            for (int i = 0; i < dim; ++i)
+                for (int j = 0; j < dim; ++j) {
+                    // C^T[j][i] = C[i][j] in column block k
+                    B[2 * (size_t(j) * W + size_t(k) * D + i)] = c[2 * (size_t(i) * dim + j)];
+                    B[2 * (size_t(j) * W + size_t(k) * D + i) + 1] = c[2 * (size_t(i) * dim + j) + 1];
+                    // (C^+ C)[i][j] = sum_r conj(C[r][i]) C[r][j]; stored transposed
+                    double re = 0.0, im = 0.0;
+                    for (int r = 0; r < dim; ++r) {
+                        const double ar = c[2 * (size_t(r) * dim + i)], ai = -c[2 * (size_t(r) * dim + i) + 1];
+                        const double br = c[2 * (size_t(r) * dim + j)], bi = c[2 * (size_t(r) * dim + j) + 1];
+                        re += ar * br - ai * bi;
+                        im += ar * bi + ai * br;
+                    }
+                    B[2 * (size_t(j) * W + size_t(n_ops + k) * D + i)] = re;
+                    B[2 * (size_t(j) * W + size_t(n_ops + k) * D + i) + 1] = im;
+                }
+        }
+        PK(p->ops.alloc(B.size() * sizeof(double)));
+        PK(hipMemcpy(p->ops.p, B.data(), B.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+#undef PK
+    *out = p.release();
+    return LLSV_OK;
+}
+
+void llsv_lplan_destroy(llsv_lplan* p) { delete p; }
+
+int llsv_lplan_run(llsv_lplan* p, const double* psi0, int64_t n_traj, int64_t first_traj, uint64_t seed, int rng,
+                   double* states) {
+    using namespace llsv;
+    if (!p) return fail(LLSV_ERR_HANDLE, "llsv_lplan_run: null plan");
+    if (!psi0 || !states || n_traj < 1 || first_traj < 0 || (rng != 0 && rng != 1))
+        return fail(LLSV_ERR_ARG, "llsv_lplan_run: bad arguments");
+    const int dim = p->dim, D = p->D, nch = p->nch;
+    const uint64_t T = uint64_t(n_traj), Tp = (T + kBM - 1) / kBM * kBM;
+    const int64_t S = p->steps;
+    const hipStream_t st = nullptr;
+#define RK(expr)                                                                                      \
+    do {                                                                                              \
+        const hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_lplan_run: %s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+    // the reference's per-trajectory streams, drawn on the host by worker threads
+    std::vector<double> table;
+    uint64_t stride = 0;
+    if (rng == 0 && nch > 0) {
+        stride = p->kind == 0 ? uint64_t(2 * S) : uint64_t(S) * nch;
+        if (double(stride) * double(T) * 8.0 > 2.0e9)
+            return fail(LLSV_ERR_ARG, "llsv_lplan_run: random-number table over 2 GB; run fewer trajectories per call");
+        table.resize(size_t(stride * T));
+        const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (unsigned w = 0; w < nth; ++w)
+            pool.emplace_back([&, w] {
+                for (uint64_t t = w; t < T; t += nth) {
+                    std::mt19937_64 mt(splitmix64(seed ^ splitmix64(uint64_t(first_traj) + t + 1)));
+                    double* o = table.data() + t * stride;
+                    if (p->kind == 0) {
+                        std::uniform_real_distribution<double> uni(0.0, 1.0);
+                        for (uint64_t c = 0; c < stride; ++c) o[c] = uni(mt);
+                    } else {
+                        std::normal_distribution<double> gauss(0.0, std::sqrt(p->h));
+                        for (uint64_t c = 0; c < stride; ++c) o[c] = gauss(mt);
+                    }
+                }
+            });
+        for (auto& th : pool) th.join();
+    }
+    const size_t sbytes = size_t(Tp) * D * 16;
+    DevBuf psi, tmp, cur, surv, flag, dtab, w1, w2, k1, k2, k3, k4, cpb;
+    RK(psi.alloc(sbytes));
+    RK(tmp.alloc(sbytes));
+    RK(cur.alloc(sizeof(uint64_t) * T));
+    RK(surv.alloc(sizeof(double) * T));
+    RK(flag.alloc(sizeof(int)));
+    RK(hipMemset(psi.p, 0, sbytes));
+    RK(hipMemset(tmp.p, 0, sbytes));
+    RK(hipMemset(cur.p, 0, sizeof(uint64_t) * T));
+    RK(hipMemset(flag.p, 0, sizeof(int)));
+    std::vector<double> hs(size_t(2) * D * T, 0.0);  // trajectories as rows of D amplitudes
+    for (uint64_t t = 0; t < T; ++t) std::memcpy(hs.data() + 2 * t * D, psi0, sizeof(double) * 2 * dim);
+    RK(hipMemcpy(psi.p, hs.data(), hs.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (!table.empty()) {
+        RK(dtab.alloc(table.size() * sizeof(double)));
+        RK(hipMemcpy(dtab.p, table.data(), table.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    Draws dr{table.empty() ? nullptr : dtab.as<const double>(), stride, seed, uint64_t(first_traj)};
+    const uint64_t nn = Tp * D;
+    const size_t blk = size_t(D) * D;
+    const v2d* nodes = p->nodes.as<const v2d>();
+    const bool constant = p->n_nodes == 1;
+    if (p->kind == 0 && !constant) {
+        RK(w1.alloc(sbytes));
+        RK(k1.alloc(sbytes));
+        RK(k2.alloc(sbytes));
+        RK(k3.alloc(sbytes));
+        RK(k4.alloc(sbytes));
+    }
+    if (p->kind == 1) {
+        RK(cpb.alloc(size_t(Tp) * std::max(1, 2 * nch) * D * 16));
+        RK(w1.alloc(sbytes));
+    }
+    const double h = p->h;
+    for (int64_t s = 0; s < S; ++s) {
+        if (p->kind == 0) {
+            if (constant) {
+                RK(gemm(st, psi.as<v2d>(), D, nodes, D, tmp.as<v2d>(), D, int(Tp), D, D));
+            } else {
+                // RK4 over H(t) sampled at the step's endpoints and midpoint (src/TrajectorySolver.h:386-396)
+                const v2d *a0 = nodes + blk * size_t(2 * s), *a1 = nodes + blk * size_t(2 * s + 1),
+                          *a2 = nodes + blk * size_t(2 * s + 2);
+                v2d* P = psi.as<v2d>();
+                RK(gemm(st, P, D, a0, D, k1.as<v2d>(), D, int(Tp), D, D));
+                hipLaunchKernelGGL(k_axpy, dim3(grid_of(nn)), dim3(256), 0, st, w1.as<v2d>(), P, k1.as<const v2d>(), 0.5 * h, nn);
+                RK(gemm(st, w1.as<v2d>(), D, a1, D, k2.as<v2d>(), D, int(Tp), D, D));
+                hipLaunchKernelGGL(k_axpy, dim3(grid_of(nn)), dim3(256), 0, st, w1.as<v2d>(), P, k2.as<const v2d>(), 0.5 * h, nn);
+                RK(gemm(st, w1.as<v2d>(), D, a1, D, k3.as<v2d>(), D, int(Tp), D, D));
+                hipLaunchKernelGGL(k_axpy, dim3(grid_of(nn)), dim3(256), 0, st, w1.as<v2d>(), P, k3.as<const v2d>(), h, nn);
+                RK(gemm(st, w1.as<v2d>(), D, a2, D, k4.as<v2d>(), D, int(Tp), D, D));
+                hipLaunchKernelGGL(k_rk4, dim3(grid_of(nn)), dim3(256), 0, st, tmp.as<v2d>(), P, k1.as<const v2d>(),
+                                   k2.as<const v2d>(), k3.as<const v2d>(), k4.as<const v2d>(), h / 6.0, nn);
+                RK(hipGetLastError());
+            }
+            hipLaunchKernelGGL(k_jump_decide, dim3(unsigned((T + 3) / 4)), dim3(256), 0, st, tmp.as<const v2d>(),
+                               psi.as<v2d>(), D, T, nch, dr, cur.as<uint64_t>(), surv.as<double>(), flag.as<int>());
+            RK(hipGetLastError());
+            if (nch > 0) {
+                hipLaunchKernelGGL(k_jump_apply, dim3(unsigned(T)), dim3(256), size_t(D) * 16, st, tmp.as<const v2d>(),
+                                   psi.as<v2d>(), D, T, nch, p->ops.as<const v2d>(), dr, cur.as<uint64_t>(),
+                                   surv.as<const double>());
+                RK(hipGetLastError());
+            }
+        } else {
+            const v2d* drift = nullptr;
+            if (constant) {
+                // exact Hamiltonian step, zero drift (src/TrajectorySolver.h:437-441)
+                RK(gemm(st, psi.as<v2d>(), D, nodes, D, tmp.as<v2d>(), D, int(Tp), D, D));
+                std::swap(psi.p, tmp.p);
+            } else {
+                RK(gemm(st, psi.as<v2d>(), D, nodes + blk * size_t(s), D, w1.as<v2d>(), D, int(Tp), D, D));
+                drift = w1.as<const v2d>();
+            }
+            if (nch > 0) RK(gemm(st, psi.as<v2d>(), D, p->ops.as<const v2d>(), 2 * nch * D, cpb.as<v2d>(), 2 * nch * D,
+                                 int(Tp), 2 * nch * D, D));
+            hipLaunchKernelGGL(k_diffuse, dim3(unsigned((T + 3) / 4)), dim3(256), 0, st, psi.as<v2d>(),
+                               cpb.as<const v2d>(), drift, D, T, nch, h, std::sqrt(h),
+                               table.empty() ? nullptr : dtab.as<const double>(), stride, uint64_t(s), dr, flag.as<int>());
+            RK(hipGetLastError());
+        }
+    }
+    int hflag = 0;
+    RK(hipMemcpy(&hflag, flag.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (hflag & 1) return fail(LLSV_ERR_NUMERIC, "TrajectorySolver: state norm vanished; reduce dt");
+    if (hflag & 2) return fail(LLSV_ERR_NUMERIC, "TrajectorySolver: diffusion state diverged; reduce dt");
+    RK(hipMemcpy(hs.data(), psi.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (uint64_t t = 0; t < T; ++t) std::memcpy(states + 2 * t * dim, hs.data() + 2 * t * D, sizeof(double) * 2 * dim);
+#undef RK
+    return LLSV_OK;
+}
+
+}  // extern "C"

@@ -16,6 +16,7 @@ LLSV_ERR_ARG = -1
 LLSV_ERR_NOMEM = -2
 LLSV_ERR_DEVICE = -3
 LLSV_ERR_HANDLE = -4
+LLSV_ERR_NUMERIC = -5
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -59,6 +60,10 @@ SIGNATURES = {
     "llsv_noisy_reseed": (_I, [_P, _U64]),
     "llsv_noisy_site": (_I, [_P, _I, ctypes.POINTER(ctypes.c_uint32)]),
     "llsv_noisy_run": (_I, [_P, _I, _P, _I, _I, ctypes.c_int64, _P, _P, _P]),
+    "llsv_expm": (_I, [_I, _P, _P]),
+    "llsv_lplan_create": (_I, [_I, _I, ctypes.c_int64, ctypes.c_double, _I, _P, _I, _P, ctypes.POINTER(_P)]),
+    "llsv_lplan_destroy": (None, [_P]),
+    "llsv_lplan_run": (_I, [_P, _P, ctypes.c_int64, ctypes.c_int64, _U64, _I, _P]),
     "llsv_pc_precompile": (_I, [_I, _P, _I, ctypes.POINTER(_I)]),
     "llsv_pc_adjoint": (_I, [_I, _I, _P, _I, _I, _P, _P, _I, _P, _P, _P, _U64, _P]),
     "llsv_traj_run": (_I, [_P, _I, _P, _I, _P, _P, _I, _U64, _P, _P, _P, _P]),
