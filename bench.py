@@ -360,6 +360,53 @@ def zne_leg(n=12, layers=4, p=0.005, trajectories=8192):
     return out
 
 
+def lindblad_leg(n=10, n_traj=1024, steps=20, dt=0.01):
+    """Dense Lindblad trajectories at 10 qubits (dim 1024): random Hermitian H, amplitude damping on
+    qubit 0 and dephasing on qubit 9, 1024 STANDARD-unraveling trajectories x 20 steps from a
+    random state, the reference's random streams.  Each step propagates every trajectory with one
+    1024 x 1024 complex GEMM (8 dim^2 T flop) on the FP64 matrix cores; timed per step with HIP
+    events around the GEMM alone, and for the whole ensemble call."""
+    import quantum_sim_trajectory as QT
+
+    d = 1 << n
+    rng = np.random.Generator(np.random.PCG64(10))
+    a = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))
+    H = (a + a.conj().T) / (2 * np.sqrt(d))
+    sm = np.array([[0, 1], [0, 0]], dtype=complex)
+    sz = np.diag([1.0, -1.0]).astype(complex)
+    s = QT.TrajectorySolver(n)
+    s.set_hamiltonian(H)
+    s.add_lindblad(np.kron(np.eye(d // 2), sm), 0.5)
+    s.add_lindblad(np.kron(sz, np.eye(d // 2)), 0.3)
+    psi0 = rng.normal(size=d) + 1j * rng.normal(size=d)
+    duration = steps * dt
+    s.evolve_ensemble(psi0, duration, dt, 64, QT.Unraveling.STANDARD, 1)  # warm-up
+    t0 = time.perf_counter()
+    rho = s.evolve_ensemble(psi0, duration, dt, n_traj, QT.Unraveling.STANDARD, 1)
+    t_all = time.perf_counter() - t0
+    # the two parts of the call: the plan (H_eff, exp(-i H_eff h) on the GPU, operator upload) and
+    # the trajectories themselves
+    t0 = time.perf_counter()
+    plan = s._plan(QT.Unraveling.STANDARD, duration, dt)
+    t_plan = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    s._run(plan, psi0 / np.linalg.norm(psi0), n_traj, 1, rho_only=True)
+    t_run = time.perf_counter() - t0
+    # the GEMM alone, as launched by a step: 1024 trajectories x 1024 x 1024
+    ms = QT.gemm_ms(n_traj, d, d, reps=20)
+    flop = 8.0 * d * d * n_traj
+    log(f"lindblad {n}q: {n_traj} trajectories x {steps} steps in {t_all:.3f} s (plan {t_plan:.3f} s, run "
+        f"{t_run:.3f} s); step GEMM {ms:.3f} ms "
+        f"= {flop / ms / 1e9:.1f} TFLOP/s")
+    return {"workload": f"{n}-qubit dense Lindblad trajectories (dim {d}), {n_traj} trajectories x {steps} steps, "
+                        "2 channels, STANDARD unraveling, reference random streams, complex128",
+            "s_per_ensemble": round(t_all, 4), "plan_s": round(t_plan, 4), "run_s": round(t_run, 4),
+            "trajectory_steps_per_s": round(n_traj * steps / t_run, 1),
+            "gemm_ms_per_step": round(ms, 4), "gemm_TFLOPs": round(flop / ms / 1e9, 2),
+            "gemm_frac_of_fp64_peak": round(flop / ms / 1e9 / 78.6, 3),
+            "trace_rho": float(np.real(np.trace(rho)))}
+
+
 def program_leg(V3, n=28, layers=20):
     """Only the program path on the 28-qubit layered circuit (for counter runs)."""
     from llsv import circuits as C, program as P
@@ -441,9 +488,9 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the latency, layered, adjoint, trajectory, ZNE and 32-qubit legs")
+                    help="skip the latency, layered, adjoint, trajectory, ZNE, Lindblad and 32-qubit legs")
     ap.add_argument("--leg", choices=["latency12", "layered28", "program28", "adjoint20", "trajectories14", "zne12",
-                                      "q32"],
+                                      "lindblad10", "q32"],
                     help="run only this leg (for profiling) and print its JSON")
     ap.add_argument("--probe", action="store_true",
                     help="launch each kernel a few times and exit (for rocprofv3 counter runs)")
@@ -464,6 +511,7 @@ def main():
         fn = {"latency12": lambda: latency_leg(V3), "layered28": lambda: layered_leg(V3),
               "program28": lambda: program_leg(V3),
               "adjoint20": adjoint_leg, "trajectories14": trajectory_leg, "zne12": zne_leg,
+              "lindblad10": lindblad_leg,
               "q32": lambda: q32_leg(V3)}[a.leg]
         print(json.dumps({a.leg: fn()}), flush=True)
         return
@@ -625,6 +673,7 @@ def main():
         line["adjoint20"] = adjoint_leg()
         line["trajectories14"] = trajectory_leg()
         line["zne12"] = zne_leg()
+        line["lindblad10"] = lindblad_leg()
         line["q32"] = q32_leg(V3)
     if not a.no_cpu_baseline:
         log("cpu baseline (subprocess)")

@@ -374,6 +374,13 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
  * (traj_detail::expm, src/TrajectorySolver.h:97-111), the products on the GPU.  Host in/out. */
 int llsv_expm(int dim, const double* a, double* out);
 
+/* C = A B for complex128 row-major matrices (host in/out), M and N multiples of 64, K of 16: the
+ * GEMM the trajectory steps run on the FP64 matrix cores.  llsv_zgemm_time: average milliseconds of
+ * `reps` back-to-back launches on device-resident operands (HIP events).  No counterpart in the
+ * reference. */
+int llsv_zgemm(int M, int N, int K, const double* a, const double* b, double* c);
+int llsv_zgemm_time(int M, int N, int K, int reps, float* ms);
+
 /* A plan (TrajectorySolver::Plan, src/TrajectorySolver.h:291-301) on the device.
  * kind 0: jumps (STANDARD / PROJECTOR; src/TrajectorySolver.h:379-428): nodes = 1 propagator
  *         exp(-i H_eff h), or 2 steps + 1 matrices -i (H(t_j) + H_nh) at t_j = j h / 2 (RK4);
@@ -387,14 +394,15 @@ int llsv_lplan_create(int dim, int kind, int64_t steps, double h, int n_nodes, c
 void llsv_lplan_destroy(llsv_lplan* p);
 
 /* Runs trajectories first_traj .. first_traj + n_traj - 1 of an ensemble from psi0 (dim complex,
- * normalised): states (host, n_traj x dim complex128) gets the final normalised states.
+ * normalised): states (host, n_traj x dim complex128, or NULL) gets the final normalised states,
+ * rho_sum (host, dim x dim complex128, or NULL) the sum of their |psi><psi| (a GEMM on the device).
  * rng 0: trajectory i draws from std::mt19937_64(splitmix64(seed ^ splitmix64(i + 1))) with
  * std::uniform_real_distribution<double>(0, 1) (jumps) or std::normal_distribution<double>(0,
  * sqrt(h)) (diffusive), like TrajectorySolver::make_rng (src/TrajectorySolver.h:311-313), so an
  * ensemble reproduces the reference's draws; rng 1: Philox4x32-10 on the device.  A vanishing or
  * diverging norm is LLSV_ERR_NUMERIC with the reference's message. */
 int llsv_lplan_run(llsv_lplan* p, const double* psi0, int64_t n_traj, int64_t first_traj, uint64_t seed, int rng,
-                   double* states);
+                   double* states, double* rho_sum);
 
 /* ---- interop and measurement ------------------------------------------------ */
 

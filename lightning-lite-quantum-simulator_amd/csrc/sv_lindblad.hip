@@ -13,8 +13,10 @@
 //   k_zgemm       tmp = prop psi                         (or the RK4 stages for a time-dependent H)
 //   k_jump_decide one wavefront per trajectory: survive = ||tmp||^2; jump iff u >= survive (u the
 //                 trajectory's next uniform); otherwise psi = tmp / sqrt(survive)
-//   k_jump_apply  one workgroup per jumping trajectory: w_k = ||C_k tmp||^2 for every channel,
-//                 channel by cumulative weight (a second uniform), psi = C_k tmp / sqrt(w_k)
+//   k_zgemm<true> the jumping trajectories' tmp rows (gathered by a device-side list) times
+//                 [C_1^T | .. | C_n^T]: every channel's C_k tmp in one GEMM
+//   k_jump_finish one wavefront per jumping trajectory: w_k = ||C_k tmp||^2, channel by cumulative
+//                 weight (a second uniform), psi = C_k tmp / sqrt(w_k)
 // The uniforms are the reference's: per trajectory std::mt19937_64 seeded with
 // splitmix64(seed ^ splitmix64(i + 1)) and std::uniform_real_distribution<double>(0, 1), drawn on
 // the host (at most two per step) -- or counter-based Philox draws on the device.
@@ -41,12 +43,23 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int kBM = 64, kBN = 64, kBK = 16;
 
+// GATHER: row r of A is row idx[r0 + r] of the A array when r0 + r < *count, zero otherwise, and
+// workgroups whose first row is at or past *count return at once (the count lives on the device).
+template <bool GATHER>
 __global__ __launch_bounds__(256) void k_zgemm(const v2d* __restrict__ A, int lda, const v2d* __restrict__ B,
-                                               int ldb, v2d* __restrict__ C, int ldc, int K) {
+                                               int ldb, v2d* __restrict__ C, int ldc, int K,
+                                               const uint32_t* __restrict__ idx, const uint32_t* __restrict__ count,
+                                               uint32_t r0) {
     __shared__ double as_r[kBK][kBM], as_i[kBK][kBM], bs_r[kBK][kBN], bs_i[kBK][kBN];
     const int tid = int(threadIdx.x), lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int m0 = int(blockIdx.y) * kBM, n0 = int(blockIdx.x) * kBN;
+    uint32_t rows = 0;
+    if (GATHER) {
+        const uint32_t cnt = *count;
+        if (r0 + uint32_t(m0) >= cnt) return;  // uniform across the workgroup
+        rows = cnt - r0 - uint32_t(m0);        // valid rows of this block (may exceed 64)
+    }
     d4 cr[2][2], ci[2][2];
     for (int a = 0; a < 2; ++a)
         for (int b = 0; b < 2; ++b) {
@@ -55,13 +68,15 @@ __global__ __launch_bounds__(256) void k_zgemm(const v2d* __restrict__ A, int ld
         }
     // tile loads: A row ar = tid / 4, columns 4 (tid % 4) .. +3; B row br = tid / 16, columns 4 (tid % 16) .. +3
     const int ar = tid >> 2, ac = (tid & 3) * 4, br = tid >> 4, bc = (tid & 15) * 4;
+    const bool a_live = !GATHER || uint32_t(ar) < rows;
+    const v2d* arow = GATHER ? (a_live ? A + size_t(idx[r0 + uint32_t(m0 + ar)]) * lda : A) : A + size_t(m0 + ar) * lda;
     for (int k0 = 0; k0 < K; k0 += kBK) {
-        const v2d* pa = A + size_t(m0 + ar) * lda + k0 + ac;
+        const v2d* pa = arow + k0 + ac;
         const v2d* pb = B + size_t(k0 + br) * ldb + n0 + bc;
         v2d va[4], vb[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            va[j] = pa[j];
+            va[j] = a_live ? pa[j] : v2d{0.0, 0.0};
             vb[j] = pb[j];
         }
         __syncthreads();  // the previous tile is no longer read
@@ -158,6 +173,16 @@ __global__ void k_rk4(v2d* __restrict__ T, const v2d* __restrict__ P, const v2d*
     }
 }
 
+// AT = X^T (D x T) and BC = conj(X) (T x D) for X of T rows of D.
+__global__ void k_transpose_conj(const v2d* __restrict__ X, v2d* __restrict__ AT, v2d* __restrict__ BC, uint64_t T, int D) {
+    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= T * uint64_t(D)) return;
+    const uint64_t t = i / D, c = i % D;
+    const v2d v = X[i];
+    AT[c * T + t] = v;
+    BC[i] = v2d{v.x, -v.y};
+}
+
 __global__ void k_identity(v2d* __restrict__ X, int D) {
     const uint64_t n = uint64_t(D) * D;
     for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += uint64_t(gridDim.x) * 256)
@@ -206,7 +231,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 // One wavefront per trajectory.  state[t]: 0 running, 1 jump pending (set here), 2 norm vanished.
 __global__ __launch_bounds__(256) void k_jump_decide(const v2d* __restrict__ tmp, v2d* __restrict__ psi, int D,
                                                      uint64_t T, int nch, Draws dr, uint64_t* __restrict__ cursor,
-                                                     double* __restrict__ survive_out, int* __restrict__ flag) {
+                                                     double* __restrict__ survive_out, int* __restrict__ flag,
+                                                     uint32_t* __restrict__ jlist, uint32_t* __restrict__ jcount) {
     const uint64_t t = uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
     const int lane = int(threadIdx.x & 63);
     if (t >= T) return;
@@ -223,8 +249,11 @@ __global__ __launch_bounds__(256) void k_jump_decide(const v2d* __restrict__ tmp
         jump = dr.u(t, c) >= s;
         if (lane == 0) cursor[t] = c + 1;
     }
-    if (lane == 0) survive_out[t] = jump ? -s : s;  // negative: jump pending
-    if (jump) return;
+    if (lane == 0) survive_out[t] = s;
+    if (jump) {
+        if (lane == 0) jlist[atomicAdd(jcount, 1u)] = uint32_t(t);
+        return;
+    }
     const double r = sqrt(s);
     v2d* y = psi + t * D;
     for (int i = lane; i < D; i += 64) {
@@ -235,50 +264,41 @@ __global__ __launch_bounds__(256) void k_jump_decide(const v2d* __restrict__ tmp
     }
 }
 
-// One workgroup per trajectory (those without a pending jump return at once).  ops: nch
-// D x D operators, row-major.
-__global__ __launch_bounds__(256) void k_jump_apply(const v2d* __restrict__ tmp, v2d* __restrict__ psi, int D,
-                                                    uint64_t T, int nch, const v2d* __restrict__ ops, Draws dr,
-                                                    uint64_t* __restrict__ cursor, const double* __restrict__ survive) {
-    extern __shared__ v2d xs[];  // tmp of this trajectory (D)
-    __shared__ double red[4];
-    __shared__ double wk[64];
-    const uint64_t t = blockIdx.x;
-    if (t >= T || !(survive[t] < 0.0)) return;
-    const int tid = int(threadIdx.x), lane = tid & 63, wave = tid >> 6;
-    for (int i = tid; i < D; i += 256) xs[i] = tmp[t * D + i];
-    __syncthreads();
-    // ||C_k tmp||^2 for every channel: one row per wavefront at a time, lanes along the row
-    for (int k = 0; k < nch; ++k) {
-        const v2d* M = ops + size_t(k) * D * D;
-        double acc = 0.0;
-        for (int r = wave; r < D; r += 4) {
-            double yr = 0.0, yi = 0.0;
-            for (int j = lane; j < D; j += 64) {
-                const v2d m = M[size_t(r) * D + j], x = xs[j];
-                yr += m.x * x.x - m.y * x.y;
-                yi += m.x * x.y + m.y * x.x;
-            }
-            yr = wave_sum(yr);
-            yi = wave_sum(yi);
-            acc += yr * yr + yi * yi;
-        }
-        if (lane == 0) red[wave] = acc;
-        __syncthreads();
-        if (tid == 0) wk[k] = (red[0] + red[1]) + (red[2] + red[3]);
-        __syncthreads();
-    }
-    // channel choice (src/TrajectorySolver.h:395-417), same in every thread
+// One wavefront per jumping trajectory r0 + r (r < rows of this round): Y row r holds
+// [C_1 tmp | .. | C_n tmp] (n D entries); w_k = ||C_k tmp||^2, the channel by cumulative weight
+// (src/TrajectorySolver.h:395-417), psi = C_k tmp / sqrt(w_k), or tmp / sqrt(survive) when no
+// channel can fire.
+__global__ __launch_bounds__(256) void k_jump_finish(const v2d* __restrict__ Y, const v2d* __restrict__ tmp,
+                                                     v2d* __restrict__ psi, int D, int nch,
+                                                     const uint32_t* __restrict__ jlist,
+                                                     const uint32_t* __restrict__ jcount, uint32_t r0, uint32_t rmax,
+                                                     Draws dr, uint64_t* __restrict__ cursor,
+                                                     const double* __restrict__ survive) {
+    const uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = int(threadIdx.x & 63);
+    const uint32_t cnt = *jcount;
+    if (r >= rmax || r0 + r >= cnt) return;
+    const uint64_t t = jlist[r0 + r];
+    const v2d* y = Y + size_t(r) * size_t(nch) * D;
+    double wk[16];
     double total = 0.0;
-    for (int k = 0; k < nch; ++k) total += wk[k];
+    for (int k = 0; k < nch; ++k) {
+        double acc = 0.0;
+        for (int i = lane; i < D; i += 64) {
+            const v2d v = y[size_t(k) * D + i];
+            acc += v.x * v.x + v.y * v.y;
+        }
+        wk[k] = wave_sum(acc);
+        total += wk[k];
+    }
+    v2d* out = psi + t * D;
     if (!(total > 0.0)) {
-        // no channel can fire: the no-jump branch
-        const double r = sqrt(-survive[t]);
-        for (int i = tid; i < D; i += 256) {
-            v2d v = xs[i];
-            v.x = v.x / r;
-            v.y = v.y / r;
-            psi[t * D + i] = v;
+        const double rs = sqrt(survive[t]);
+        for (int i = lane; i < D; i += 64) {
+            v2d v = tmp[t * D + i];
+            v.x = v.x / rs;
+            v.y = v.y / rs;
+            out[i] = v;
         }
         return;
     }
@@ -299,20 +319,13 @@ __global__ __launch_bounds__(256) void k_jump_apply(const v2d* __restrict__ tmp,
                 pick = k;
                 break;
             }
-    __syncthreads();
-    if (tid == 0) cursor[t] = c + 1;
+    if (lane == 0) cursor[t] = c + 1;
     const double nrm = sqrt(wk[pick]);
-    const v2d* M = ops + size_t(pick) * D * D;
-    for (int r = wave; r < D; r += 4) {
-        double yr = 0.0, yi = 0.0;
-        for (int j = lane; j < D; j += 64) {
-            const v2d m = M[size_t(r) * D + j], x = xs[j];
-            yr += m.x * x.x - m.y * x.y;
-            yi += m.x * x.y + m.y * x.x;
-        }
-        yr = wave_sum(yr);
-        yi = wave_sum(yi);
-        if (lane == 0) psi[t * D + r] = v2d{yr / nrm, yi / nrm};
+    for (int i = lane; i < D; i += 64) {
+        v2d v = y[size_t(pick) * D + i];
+        v.x = v.x / nrm;
+        v.y = v.y / nrm;
+        out[i] = v;
     }
 }
 
@@ -381,7 +394,16 @@ __global__ __launch_bounds__(256) void k_diffuse(v2d* __restrict__ psi, const v2
 int pad_dim(int dim) { return dim < kBN ? kBN : dim; }
 
 hipError_t gemm(hipStream_t st, const v2d* A, int lda, const v2d* B, int ldb, v2d* C, int ldc, int M, int N, int K) {
-    hipLaunchKernelGGL(k_zgemm, dim3(unsigned(N / kBN), unsigned(M / kBM)), dim3(256), 0, st, A, lda, B, ldb, C, ldc, K);
+    hipLaunchKernelGGL(k_zgemm<false>, dim3(unsigned(N / kBN), unsigned(M / kBM)), dim3(256), 0, st, A, lda, B, ldb, C,
+                       ldc, K, nullptr, nullptr, 0u);
+    return hipGetLastError();
+}
+
+// C = A[idx[r0 ..]] B over at most M gathered rows; blocks past the device-side count return.
+hipError_t gemm_gather(hipStream_t st, const v2d* A, int lda, const uint32_t* idx, const uint32_t* count, uint32_t r0,
+                       const v2d* B, int ldb, v2d* C, int ldc, int M, int N, int K) {
+    hipLaunchKernelGGL(k_zgemm<true>, dim3(unsigned(N / kBN), unsigned(M / kBM)), dim3(256), 0, st, A, lda, B, ldb, C,
+                       ldc, K, idx, count, r0);
     return hipGetLastError();
 }
 
@@ -489,6 +511,54 @@ int llsv_expm(int dim, const double* a, double* out) {
     return LLSV_OK;
 }
 
+int llsv_zgemm(int M, int N, int K, const double* a, const double* b, double* c) {
+    using namespace llsv;
+    if (M < 1 || N < 1 || K < 1 || !a || !b || !c) return fail(LLSV_ERR_ARG, "llsv_zgemm: bad arguments");
+    if (M % kBM || N % kBN || K % kBK)
+        return fail(LLSV_ERR_ARG, "llsv_zgemm: M, N must be multiples of 64 and K of 16");
+    DevBuf da, db, dc;
+#define GK(expr)                                                                                      \
+    do {                                                                                              \
+        const hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_zgemm: %s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+    GK(da.alloc(size_t(M) * K * 16));
+    GK(db.alloc(size_t(K) * N * 16));
+    GK(dc.alloc(size_t(M) * N * 16));
+    GK(hipMemcpy(da.p, a, size_t(M) * K * 16, hipMemcpyHostToDevice));
+    GK(hipMemcpy(db.p, b, size_t(K) * N * 16, hipMemcpyHostToDevice));
+    GK(gemm(nullptr, da.as<v2d>(), K, db.as<v2d>(), N, dc.as<v2d>(), N, M, N, K));
+    GK(hipMemcpy(c, dc.p, size_t(M) * N * 16, hipMemcpyDeviceToHost));
+    return LLSV_OK;
+}
+
+int llsv_zgemm_time(int M, int N, int K, int reps, float* ms) {
+    using namespace llsv;
+    if (M < 1 || N < 1 || K < 1 || reps < 1 || !ms || M % kBM || N % kBN || K % kBK)
+        return fail(LLSV_ERR_ARG, "llsv_zgemm_time: bad arguments");
+    DevBuf da, db, dc;
+    GK(da.alloc(size_t(M) * K * 16));
+    GK(db.alloc(size_t(K) * N * 16));
+    GK(dc.alloc(size_t(M) * N * 16));
+    GK(hipMemset(da.p, 0, size_t(M) * K * 16));
+    GK(hipMemset(db.p, 0, size_t(K) * N * 16));
+    hipEvent_t e0, e1;
+    GK(hipEventCreate(&e0));
+    GK(hipEventCreate(&e1));
+    GK(gemm(nullptr, da.as<v2d>(), K, db.as<v2d>(), N, dc.as<v2d>(), N, M, N, K));  // warm-up
+    GK(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < reps; ++r) GK(gemm(nullptr, da.as<v2d>(), K, db.as<v2d>(), N, dc.as<v2d>(), N, M, N, K));
+    GK(hipEventRecord(e1, nullptr));
+    GK(hipEventSynchronize(e1));
+    float t = 0.0f;
+    GK(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+#undef GK
+    *ms = t / float(reps);
+    return LLSV_OK;
+}
+
 int llsv_lplan_create(int dim, int kind, int64_t steps, double h, int n_nodes, const double* nodes, int n_ops,
                       const double* ops, llsv_lplan** out) {
     using namespace llsv;
@@ -522,36 +592,49 @@ int llsv_lplan_create(int dim, int kind, int64_t steps, double h, int n_nodes, c
         PK(hipMemcpy(p->nodes.as<char>() + blk * k, m.data(), blk, hipMemcpyHostToDevice));
     }
     if (kind == 0) {
-        PK(p->ops.alloc(blk * size_t(std::max(1, n_ops))));
-        for (int k = 0; k < n_ops; ++k) {
-            const std::vector<double> m = padded(ops + 2 * size_t(k) * dim * dim, dim, D, false);
-            PK(hipMemcpy(p->ops.as<char>() + blk * k, m.data(), blk, hipMemcpyHostToDevice));
-        }
-    } else {
-        // B = [C_1^T | .. | C_n^T | D_1^T | .. | D_n^T], D x 2nD, D_k = C_k^+ C_k (computed here)
-        const int W = 2 * n_ops * D;
+        // B = [C_1^T | .. | C_n^T], D x nD: one gathered GEMM gives every channel's C_k tmp
+        const int W = n_ops * D;
         std::vector<double> B(size_t(2) * D * std::max(W, 1), 0.0);
         for (int k = 0; k < n_ops; ++k) {
             const double* c = ops + 2 * size_t(k) * dim * dim;
             for (int i = 0; i < dim; ++i)
                 for (int j = 0; j < dim; ++j) {
-                    // C^T[j][i] = C[i][j] in column block k
                     B[2 * (size_t(j) * W + size_t(k) * D + i)] = c[2 * (size_t(i) * dim + j)];
                     B[2 * (size_t(j) * W + size_t(k) * D + i) + 1] = c[2 * (size_t(i) * dim + j) + 1];
-                    // (C^+ C)[i][j] = sum_r conj(C[r][i]) C[r][j]; stored transposed
-                    double re = 0.0, im = 0.0;
-                    for (int r = 0; r < dim; ++r) {
-                        const double ar = c[2 * (size_t(r) * dim + i)], ai = -c[2 * (size_t(r) * dim + i) + 1];
-                        const double br = c[2 * (size_t(r) * dim + j)], bi = c[2 * (size_t(r) * dim + j) + 1];
-                        re += ar * br - ai * bi;
-                        im += ar * bi + ai * br;
-                    }
-                    B[2 * (size_t(j) * W + size_t(n_ops + k) * D + i)] = re;
-                    B[2 * (size_t(j) * W + size_t(n_ops + k) * D + i) + 1] = im;
                 }
         }
         PK(p->ops.alloc(B.size() * sizeof(double)));
         PK(hipMemcpy(p->ops.p, B.data(), B.size() * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        // B = [C_1^T | .. | C_n^T | D_1^T | .. | D_n^T], D x 2nD, D_k = C_k^+ C_k; the products
+        // D_k^T = C_k^T conj(C_k) are GEMMs on the device, written into their column blocks
+        const int W = 2 * n_ops * D;
+        std::vector<double> B(size_t(2) * D * std::max(W, 1), 0.0);
+        std::vector<double> cj(size_t(2) * D * D * std::max(n_ops, 1), 0.0);  // conj(C_k), padded
+        for (int k = 0; k < n_ops; ++k) {
+            const double* c = ops + 2 * size_t(k) * dim * dim;
+            double* q = cj.data() + 2 * size_t(k) * D * D;
+            for (int i = 0; i < dim; ++i)
+                for (int j = 0; j < dim; ++j) {
+                    // C^T[j][i] = C[i][j] in column block k
+                    B[2 * (size_t(j) * W + size_t(k) * D + i)] = c[2 * (size_t(i) * dim + j)];
+                    B[2 * (size_t(j) * W + size_t(k) * D + i) + 1] = c[2 * (size_t(i) * dim + j) + 1];
+                    q[2 * (size_t(i) * D + j)] = c[2 * (size_t(i) * dim + j)];
+                    q[2 * (size_t(i) * D + j) + 1] = -c[2 * (size_t(i) * dim + j) + 1];
+                }
+        }
+        PK(p->ops.alloc(B.size() * sizeof(double)));
+        PK(hipMemcpy(p->ops.p, B.data(), B.size() * sizeof(double), hipMemcpyHostToDevice));
+        if (n_ops > 0) {
+            DevBuf dcj;
+            PK(dcj.alloc(cj.size() * sizeof(double)));
+            PK(hipMemcpy(dcj.p, cj.data(), cj.size() * sizeof(double), hipMemcpyHostToDevice));
+            v2d* stack = p->ops.as<v2d>();
+            for (int k = 0; k < n_ops; ++k)
+                PK(gemm(nullptr, stack + size_t(k) * D, W, dcj.as<const v2d>() + size_t(k) * D * D, D,
+                        stack + size_t(n_ops + k) * D, W, D, D, D));
+            PK(hipDeviceSynchronize());
+        }
     }
 #undef PK
     *out = p.release();
@@ -561,10 +644,10 @@ int llsv_lplan_create(int dim, int kind, int64_t steps, double h, int n_nodes, c
 void llsv_lplan_destroy(llsv_lplan* p) { delete p; }
 
 int llsv_lplan_run(llsv_lplan* p, const double* psi0, int64_t n_traj, int64_t first_traj, uint64_t seed, int rng,
-                   double* states) {
+                   double* states, double* rho_sum) {
     using namespace llsv;
     if (!p) return fail(LLSV_ERR_HANDLE, "llsv_lplan_run: null plan");
-    if (!psi0 || !states || n_traj < 1 || first_traj < 0 || (rng != 0 && rng != 1))
+    if (!psi0 || (!states && !rho_sum) || n_traj < 1 || first_traj < 0 || (rng != 0 && rng != 1))
         return fail(LLSV_ERR_ARG, "llsv_lplan_run: bad arguments");
     const int dim = p->dim, D = p->D, nch = p->nch;
     const uint64_t T = uint64_t(n_traj), Tp = (T + kBM - 1) / kBM * kBM;
@@ -602,7 +685,7 @@ int llsv_lplan_run(llsv_lplan* p, const double* psi0, int64_t n_traj, int64_t fi
         for (auto& th : pool) th.join();
     }
     const size_t sbytes = size_t(Tp) * D * 16;
-    DevBuf psi, tmp, cur, surv, flag, dtab, w1, w2, k1, k2, k3, k4, cpb;
+    DevBuf psi, tmp, cur, surv, flag, dtab, w1, w2, k1, k2, k3, k4, cpb, jl, jc, ybuf;
     RK(psi.alloc(sbytes));
     RK(tmp.alloc(sbytes));
     RK(cur.alloc(sizeof(uint64_t) * T));
@@ -620,6 +703,16 @@ int llsv_lplan_run(llsv_lplan* p, const double* psi0, int64_t n_traj, int64_t fi
         RK(hipMemcpy(dtab.p, table.data(), table.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     Draws dr{table.empty() ? nullptr : dtab.as<const double>(), stride, seed, uint64_t(first_traj)};
+    // jumps: the list of this step's jumping trajectories and the gathered products, processed in
+    // rounds of at most R rows (Y of R x nD entries, at most 512 MiB)
+    uint32_t R = 0;
+    if (p->kind == 0 && nch > 0) {
+        RK(jl.alloc(sizeof(uint32_t) * T));
+        RK(jc.alloc(sizeof(uint32_t)));
+        const uint64_t row_bytes = uint64_t(nch) * D * 16;
+        R = uint32_t(std::min<uint64_t>(Tp, std::max<uint64_t>(kBM, ((uint64_t(512) << 20) / row_bytes) / kBM * kBM)));
+        RK(ybuf.alloc(size_t(R) * row_bytes));
+    }
     const uint64_t nn = Tp * D;
     const size_t blk = size_t(D) * D;
     const v2d* nodes = p->nodes.as<const v2d>();
@@ -656,13 +749,18 @@ int llsv_lplan_run(llsv_lplan* p, const double* psi0, int64_t n_traj, int64_t fi
                                    k2.as<const v2d>(), k3.as<const v2d>(), k4.as<const v2d>(), h / 6.0, nn);
                 RK(hipGetLastError());
             }
+            if (nch > 0) RK(hipMemsetAsync(jc.p, 0, sizeof(uint32_t), st));
             hipLaunchKernelGGL(k_jump_decide, dim3(unsigned((T + 3) / 4)), dim3(256), 0, st, tmp.as<const v2d>(),
-                               psi.as<v2d>(), D, T, nch, dr, cur.as<uint64_t>(), surv.as<double>(), flag.as<int>());
+                               psi.as<v2d>(), D, T, nch, dr, cur.as<uint64_t>(), surv.as<double>(), flag.as<int>(),
+                               jl.as<uint32_t>(), jc.as<uint32_t>());
             RK(hipGetLastError());
-            if (nch > 0) {
-                hipLaunchKernelGGL(k_jump_apply, dim3(unsigned(T)), dim3(256), size_t(D) * 16, st, tmp.as<const v2d>(),
-                                   psi.as<v2d>(), D, T, nch, p->ops.as<const v2d>(), dr, cur.as<uint64_t>(),
-                                   surv.as<const double>());
+            for (uint32_t r0 = 0; nch > 0 && uint64_t(r0) < T; r0 += R) {
+                // rounds past the device-side count return at once
+                RK(gemm_gather(st, tmp.as<const v2d>(), D, jl.as<const uint32_t>(), jc.as<const uint32_t>(), r0,
+                               p->ops.as<const v2d>(), nch * D, ybuf.as<v2d>(), nch * D, int(R), nch * D, D));
+                hipLaunchKernelGGL(k_jump_finish, dim3((R + 3) / 4), dim3(256), 0, st, ybuf.as<const v2d>(),
+                                   tmp.as<const v2d>(), psi.as<v2d>(), D, nch, jl.as<const uint32_t>(),
+                                   jc.as<const uint32_t>(), r0, R, dr, cur.as<uint64_t>(), surv.as<const double>());
                 RK(hipGetLastError());
             }
         } else {
@@ -687,8 +785,24 @@ int llsv_lplan_run(llsv_lplan* p, const double* psi0, int64_t n_traj, int64_t fi
     RK(hipMemcpy(&hflag, flag.p, sizeof(int), hipMemcpyDeviceToHost));
     if (hflag & 1) return fail(LLSV_ERR_NUMERIC, "TrajectorySolver: state norm vanished; reduce dt");
     if (hflag & 2) return fail(LLSV_ERR_NUMERIC, "TrajectorySolver: diffusion state diverged; reduce dt");
-    RK(hipMemcpy(hs.data(), psi.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (uint64_t t = 0; t < T; ++t) std::memcpy(states + 2 * t * dim, hs.data() + 2 * t * D, sizeof(double) * 2 * dim);
+    if (states) {
+        RK(hipMemcpy(hs.data(), psi.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (uint64_t t = 0; t < T; ++t) std::memcpy(states + 2 * t * dim, hs.data() + 2 * t * D, sizeof(double) * 2 * dim);
+    }
+    if (rho_sum) {
+        // sum_t psi_t psi_t^+ = Psi^T conj(Psi) (trajectories as rows; padded rows are zero)
+        DevBuf at, bc, rs;
+        RK(at.alloc(sbytes));
+        RK(bc.alloc(sbytes));
+        RK(rs.alloc(size_t(D) * D * 16));
+        hipLaunchKernelGGL(k_transpose_conj, dim3(unsigned((Tp * D + 255) / 256)), dim3(256), 0, st, psi.as<const v2d>(),
+                           at.as<v2d>(), bc.as<v2d>(), Tp, D);
+        RK(hipGetLastError());
+        RK(gemm(st, at.as<const v2d>(), int(Tp), bc.as<const v2d>(), D, rs.as<v2d>(), D, D, D, int(Tp)));
+        std::vector<double> hr(size_t(2) * D * D);
+        RK(hipMemcpy(hr.data(), rs.p, hr.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < dim; ++i) std::memcpy(rho_sum + 2 * size_t(i) * dim, hr.data() + 2 * size_t(i) * D, sizeof(double) * 2 * dim);
+    }
 #undef RK
     return LLSV_OK;
 }

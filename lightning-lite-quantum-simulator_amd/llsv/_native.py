@@ -60,10 +60,12 @@ SIGNATURES = {
     "llsv_noisy_reseed": (_I, [_P, _U64]),
     "llsv_noisy_site": (_I, [_P, _I, ctypes.POINTER(ctypes.c_uint32)]),
     "llsv_noisy_run": (_I, [_P, _I, _P, _I, _I, ctypes.c_int64, _P, _P, _P]),
+    "llsv_zgemm": (_I, [_I, _I, _I, _P, _P, _P]),
+    "llsv_zgemm_time": (_I, [_I, _I, _I, _I, ctypes.POINTER(ctypes.c_float)]),
     "llsv_expm": (_I, [_I, _P, _P]),
     "llsv_lplan_create": (_I, [_I, _I, ctypes.c_int64, ctypes.c_double, _I, _P, _I, _P, ctypes.POINTER(_P)]),
     "llsv_lplan_destroy": (None, [_P]),
-    "llsv_lplan_run": (_I, [_P, _P, ctypes.c_int64, ctypes.c_int64, _U64, _I, _P]),
+    "llsv_lplan_run": (_I, [_P, _P, ctypes.c_int64, ctypes.c_int64, _U64, _I, _P, _P]),
     "llsv_pc_precompile": (_I, [_I, _P, _I, ctypes.POINTER(_I)]),
     "llsv_pc_adjoint": (_I, [_I, _I, _P, _I, _I, _P, _P, _I, _P, _P, _P, _U64, _P]),
     "llsv_traj_run": (_I, [_P, _I, _P, _I, _P, _P, _I, _U64, _P, _P, _P, _P]),

@@ -41,13 +41,6 @@ def _fail(who, msg):
     raise ValueError(f"{who}: {msg}")
 
 
-def _cmat(m, dim, who, what="operator must be dim x dim"):
-    a = np.asarray(m, dtype=np.complex128)
-    if a.shape != (dim, dim):
-        _fail(who, what)
-    return np.ascontiguousarray(a)
-
-
 def expm(a) -> np.ndarray:
     """exp(A) by the reference's scaling and squaring (src/TrajectorySolver.h:97-111), on the GPU."""
     a = np.ascontiguousarray(a, dtype=np.complex128)
@@ -55,6 +48,30 @@ def expm(a) -> np.ndarray:
     _native.check(_native.lib().llsv_expm(a.shape[0], a.ctypes.data_as(ctypes.c_void_p),
                                           out.ctypes.data_as(ctypes.c_void_p)))
     return out
+
+
+def zgemm(a, b) -> np.ndarray:
+    """a @ b on the trajectory GEMM kernel (rows and columns multiples of 64, inner size of 16)."""
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    b = np.ascontiguousarray(b, dtype=np.complex128)
+    c = np.empty((a.shape[0], b.shape[1]), dtype=np.complex128)
+    _native.check(_native.lib().llsv_zgemm(a.shape[0], b.shape[1], a.shape[1], a.ctypes.data_as(ctypes.c_void_p),
+                                           b.ctypes.data_as(ctypes.c_void_p), c.ctypes.data_as(ctypes.c_void_p)))
+    return c
+
+
+def _matmul(a, b):
+    """a @ b: on the GPU kernel from 64 x 64 up (dims are powers of two), NumPy below."""
+    if a.shape[0] >= 64 and a.shape[0] % 64 == 0 and b.shape[1] % 64 == 0 and a.shape[1] % 16 == 0:
+        return zgemm(a, b)
+    return a @ b
+
+
+def gemm_ms(M: int, N: int, K: int, reps: int = 20) -> float:
+    """Average device milliseconds of one M x K by K x N complex GEMM launch (HIP events)."""
+    ms = ctypes.c_float(0.0)
+    _native.check(_native.lib().llsv_zgemm_time(M, N, K, reps, ctypes.byref(ms)))
+    return float(ms.value)
 
 
 class TrajectorySolver:
@@ -132,7 +149,7 @@ class TrajectorySolver:
         """L Hermitian with L^2 = c 1, c > 0 (src/TrajectorySolver.h:316-326)."""
         if np.linalg.norm(l - l.conj().T) > 1e-9 * max(1.0, np.linalg.norm(l)):
             return False, None, 0.0
-        l2 = l @ l
+        l2 = _matmul(l, l)
         c = float(l2[0, 0].real)
         if not c > 1e-12:
             return False, None, 0.0
@@ -198,7 +215,7 @@ class TrajectorySolver:
         hnh = np.zeros((dim, dim), dtype=np.complex128)
         if jump:
             for j in ops:
-                hnh = hnh - 0.5j * (j.conj().T @ j)
+                hnh = hnh - 0.5j * _matmul(j.conj().T, j)
         if self._h_fn is None:
             nodes = [expm((-1j) * (self._h_const + hnh) * h)]
         else:
@@ -218,19 +235,26 @@ class TrajectorySolver:
                                             opsa.ctypes.data_as(ctypes.c_void_p), ctypes.byref(p)))
         return _Plan(lib, p, steps, jump)
 
-    def _run(self, plan, psi0, n, seed):
-        """Final states of trajectories 0 .. n-1 (n x dim), batched so the host random table stays small."""
-        out = np.empty((n, self._dim), dtype=np.complex128)
+    def _run(self, plan, psi0, n, seed, rho_only=False):
+        """Final states of trajectories 0 .. n-1 (n x dim), or with rho_only the sum of their
+        |psi><psi| (dim x dim, formed on the device); batched so the host random table stays small."""
+        d = self._dim
+        out = None if rho_only else np.empty((n, d), dtype=np.complex128)
+        acc = np.zeros((d, d), dtype=np.complex128) if rho_only else None
+        part = np.empty((d, d), dtype=np.complex128) if rho_only else None
         per = (2 * plan.steps if plan.jump else plan.steps * max(1, self.n_channels)) * 8
         batch = max(1, min(n, (256 << 20) // max(per, 1)))
         psi0 = np.ascontiguousarray(psi0, dtype=np.complex128)
         rng = 0 if self.rng == "reference" else 1
         for f in range(0, n, batch):
             b = min(batch, n - f)
+            st = None if rho_only else out[f:f + b].ctypes.data_as(ctypes.c_void_p)
+            rs = part.ctypes.data_as(ctypes.c_void_p) if rho_only else None
             _native.check(plan.lib.llsv_lplan_run(plan.h, psi0.ctypes.data_as(ctypes.c_void_p), b, f,
-                                                  int(seed) & ((1 << 64) - 1), rng,
-                                                  out[f:f + b].ctypes.data_as(ctypes.c_void_p)))
-        return out
+                                                  int(seed) & ((1 << 64) - 1), rng, st, rs))
+            if rho_only:
+                acc += part
+        return acc if rho_only else out
 
     # -- evolution (src/TrajectorySolver.h:158-256)
     def run_one(self, psi0, duration, dt, seed, kind=Unraveling.STANDARD):
@@ -251,8 +275,7 @@ class TrajectorySolver:
             _fail(who, "n_trajectories must be >= 1")
         if duration == 0.0:
             return np.outer(start, start.conj())
-        states = self._run(self._plan(kind, duration, dt), start, n, seed)
-        return (states.T @ states.conj()) / float(n)
+        return self._run(self._plan(kind, duration, dt), start, n, seed, rho_only=True) / float(n)
 
     def estimate_observable(self, psi0, duration, dt, n_trajectories, obs, kind=Unraveling.STANDARD, seed=1):
         who = "estimate_observable"

@@ -46,6 +46,19 @@ def test_expm_vs_scipy(d, scale):
     assert np.max(np.abs(got - LB.expm(a))) < 1e-12 * max(1.0, np.max(np.abs(ref)))
 
 
+@pytest.mark.parametrize("M,N,K", [(64, 64, 16), (128, 192, 48), (256, 64, 1024)])
+def test_zgemm_vs_numpy(M, N, K):
+    import quantum_sim_trajectory as QT
+
+    rng = np.random.default_rng(M + N + K)
+    a = rng.normal(size=(M, K)) + 1j * rng.normal(size=(M, K))
+    b = rng.normal(size=(K, N)) + 1j * rng.normal(size=(K, N))
+    ref = a @ b
+    assert np.max(np.abs(QT.zgemm(a, b) - ref)) < 1e-12 * K
+    with pytest.raises(ValueError):
+        QT.zgemm(a[:, :8], b[:8])
+
+
 def two_qubit_model():
     H = np.kron(SX, SX) + 0.3 * np.kron(SZ, np.eye(2)) + 0.2 * np.kron(np.eye(2), SY)
     chans = [(np.kron(np.eye(2), SZ), 0.2), (np.kron(SM, np.eye(2)), 0.4), (np.kron(SX, SX), 0.1)]
