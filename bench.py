@@ -360,12 +360,12 @@ def zne_leg(n=12, layers=4, p=0.005, trajectories=8192):
     return out
 
 
-def lindblad_leg(n=10, n_traj=1024, steps=20, dt=0.01):
+def lindblad_leg(n=10, n_traj=4096, steps=20, dt=0.01):
     """Dense Lindblad trajectories at 10 qubits (dim 1024): random Hermitian H, amplitude damping on
-    qubit 0 and dephasing on qubit 9, 1024 STANDARD-unraveling trajectories x 20 steps from a
+    qubit 0 and dephasing on qubit 9, 4096 STANDARD-unraveling trajectories x 20 steps from a
     random state, the reference's random streams.  Each step propagates every trajectory with one
-    1024 x 1024 complex GEMM (8 dim^2 T flop) on the FP64 matrix cores; timed per step with HIP
-    events around the GEMM alone, and for the whole ensemble call."""
+    4096 x 1024 by 1024 x 1024 complex GEMM (8 dim^2 T flop) on the FP64 matrix cores; timed per
+    step with HIP events around the GEMM alone, and for the whole ensemble call."""
     import quantum_sim_trajectory as QT
 
     d = 1 << n
@@ -392,7 +392,7 @@ def lindblad_leg(n=10, n_traj=1024, steps=20, dt=0.01):
     t0 = time.perf_counter()
     s._run(plan, psi0 / np.linalg.norm(psi0), n_traj, 1, rho_only=True)
     t_run = time.perf_counter() - t0
-    # the GEMM alone, as launched by a step: 1024 trajectories x 1024 x 1024
+    # the GEMM alone, as launched by a step: n_traj trajectories x 1024 x 1024
     ms = QT.gemm_ms(n_traj, d, d, reps=20)
     flop = 8.0 * d * d * n_traj
     log(f"lindblad {n}q: {n_traj} trajectories x {steps} steps in {t_all:.3f} s (plan {t_plan:.3f} s, run "

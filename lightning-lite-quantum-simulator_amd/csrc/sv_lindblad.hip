@@ -70,15 +70,19 @@ __global__ __launch_bounds__(256) void k_zgemm(const v2d* __restrict__ A, int ld
     const int ar = tid >> 2, ac = (tid & 3) * 4, br = tid >> 4, bc = (tid & 15) * 4;
     const bool a_live = !GATHER || uint32_t(ar) < rows;
     const v2d* arow = GATHER ? (a_live ? A + size_t(idx[r0 + uint32_t(m0 + ar)]) * lda : A) : A + size_t(m0 + ar) * lda;
-    for (int k0 = 0; k0 < K; k0 += kBK) {
+    // software pipeline: the next tile's global loads are in flight while this tile is multiplied
+    v2d va[4], vb[4];
+    auto load = [&](int k0) {
         const v2d* pa = arow + k0 + ac;
         const v2d* pb = B + size_t(k0 + br) * ldb + n0 + bc;
-        v2d va[4], vb[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             va[j] = a_live ? pa[j] : v2d{0.0, 0.0};
             vb[j] = pb[j];
         }
+    };
+    load(0);
+    for (int k0 = 0; k0 < K; k0 += kBK) {
         __syncthreads();  // the previous tile is no longer read
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -88,6 +92,7 @@ __global__ __launch_bounds__(256) void k_zgemm(const v2d* __restrict__ A, int ld
             bs_i[br][bc + j] = vb[j].y;
         }
         __syncthreads();
+        if (k0 + kBK < K) load(k0 + kBK);
 #pragma unroll
         for (int ks = 0; ks < kBK / 4; ++ks) {
             const int kk = ks * 4 + (lane >> 4);
