@@ -34,8 +34,114 @@ std::atomic<uint64_t> g_compiled{0}, g_launches{0}, g_failures{0};
 
 const char kPrelude[] =
     "typedef double v2d __attribute__((ext_vector_type(2)));\n"
-    "struct PassDesc { int n, b, L, nops; int tq[12]; int pos[12]; int rq[4]; };\n" LLSV_STRINGIZE(
-        LLSV_REGOPS_SOURCE) "\n";
+    "struct PassDesc { int n, b, L, nops; int tq[12]; int pos[12]; int rq[4]; };\n"
+    "__device__ __forceinline__ unsigned int r_swzm(unsigned int l, unsigned int r0, unsigned int r1,"
+    " unsigned int r2, unsigned int r3) { return l ^ ((__builtin_popcount(l & r0) & 1u) |"
+    " ((__builtin_popcount(l & r1) & 1u) << 1) | ((__builtin_popcount(l & r2) & 1u) << 2) |"
+    " ((__builtin_popcount(l & r3) & 1u) << 3)); }\n" LLSV_STRINGIZE(LLSV_REGOPS_SOURCE) "\n";
+
+// ---- LDS slot swizzle of a generated pass ----------------------------------------------------
+// A layout change writes every amplitude of the tile into LDS and reads it back in the new
+// layout; amplitude L (12-bit local index) sits in 16-byte slot f(L), f(L) = L ^ (M L) with M a
+// 4 x 12 matrix over GF(2) into bits 0..3 (rows[b] = the bits whose parity flips slot bit b).  f
+// is linear, so a lane's slot is f(its lane part) ^ f(the register part).  A 128-bit LDS access
+// serves 8 (or 16) consecutive lanes per cycle, so it is free of bank conflicts when the lowest 3
+// (or 4) lane positions of the layout map to independent slot bits mod 8 (or 16).  The fixed
+// swizzle of the interpreter (fold bits 4..7 onto 0..3) fails that for some register sets; a
+// generated pass picks its own M: the one, among the fixed fold and a deterministic family of
+// candidates, with the fewest conflict cycles over every layout of the pass.
+struct Swz {
+    uint32_t row[4];
+};
+
+uint32_t swz_apply(const Swz& m, uint32_t l) {
+    uint32_t o = 0;
+    for (int b = 0; b < 4; ++b) o |= uint32_t(__builtin_popcount(l & m.row[b]) & 1) << b;
+    return l ^ o;
+}
+
+// Local index bits of register j in register set S (deposit the 4 bits of j at S's positions).
+uint32_t reg_bits(uint32_t S, int j) {
+    uint32_t o = 0;
+    int t = 0;
+    for (int bit = 0; bit < kRegTileBits; ++bit)
+        if ((S >> bit) & 1u) {
+            if ((j >> t) & 1) o |= 1u << bit;
+            ++t;
+        }
+    return o;
+}
+
+int rank_gf2(uint32_t* v, int n) {
+    int r = 0;
+    for (int bit = 3; bit >= 0; --bit) {
+        int piv = -1;
+        for (int i = r; i < n; ++i)
+            if ((v[i] >> bit) & 1u) {
+                piv = i;
+                break;
+            }
+        if (piv < 0) continue;
+        std::swap(v[r], v[piv]);
+        for (int i = 0; i < n; ++i)
+            if (i != r && ((v[i] >> bit) & 1u)) v[i] ^= v[r];
+        ++r;
+    }
+    return r;
+}
+
+// Extra cycles of the 128-bit accesses made with register set S (lanes deposit tid into the other
+// positions, ascending), for 8- and 16-lane service groups.
+int swz_cost(const Swz& m, uint32_t S) {
+    int pos[8], k = 0;
+    for (int bit = 0; bit < kRegTileBits && k < 8; ++bit)
+        if (!((S >> bit) & 1u)) pos[k++] = bit;
+    uint32_t v16[4], v8[3];
+    for (int i = 0; i < 4; ++i) v16[i] = swz_apply(m, 1u << pos[i]) & 15u;
+    for (int i = 0; i < 3; ++i) v8[i] = swz_apply(m, 1u << pos[i]) & 7u;
+    return ((1 << (4 - rank_gf2(v16, 4))) - 1) + ((1 << (3 - rank_gf2(v8, 3))) - 1);
+}
+
+bool swz_invertible(const Swz& m) {
+    uint32_t v[4];
+    for (int c = 0; c < 4; ++c) v[c] = swz_apply(m, 1u << c) & 15u;
+    return rank_gf2(v, 4) == 4;
+}
+
+Swz choose_swizzle(const RegOp* ops, int nops) {
+    std::vector<uint32_t> sets;  // register sets of every LDS access (both sides of each layout change)
+    uint32_t S = 0xF00u;
+    for (int i = 0; i < nops; ++i)
+        if (ops[i].kind == RO_LAYOUT) {
+            sets.push_back(S);
+            S = (1u << ops[i].q[0]) | (1u << ops[i].q[1]) | (1u << ops[i].q[2]) | (1u << ops[i].q[3]);
+            sets.push_back(S);
+        }
+    Swz best{{0x10u, 0x20u, 0x40u, 0x80u}};  // the interpreter's fold
+    auto cost = [&](const Swz& m) {
+        int c = 0;
+        for (uint32_t x : sets) c += swz_cost(m, x);
+        return c;
+    };
+    int best_cost = cost(best);
+    uint64_t st = 0x9E3779B97F4A7C15ull;  // fixed seed: the same pass always gets the same swizzle
+    for (int it = 0; it < 2048 && best_cost > 0; ++it) {
+        Swz c;
+        for (int b = 0; b < 4; ++b) {
+            st ^= st << 13;
+            st ^= st >> 7;
+            st ^= st << 17;
+            c.row[b] = uint32_t(st) & 0xFFFu;
+        }
+        if (!swz_invertible(c)) continue;
+        const int k = cost(c);
+        if (k < best_cost) {
+            best = c;
+            best_cost = k;
+        }
+    }
+    return best;
+}
 
 struct Src {
     std::string s;
@@ -85,6 +191,9 @@ struct Shape {
     bool two = false;      // a second state (lambda) goes through every op; RO_GRAD ops reduce into partials
 };
 
+// Every generated kernel asks for two waves per SIMD (__launch_bounds__(256, 2)): with the 64 KiB
+// exchange buffer two workgroups fit a CU, and a kernel allowed more than 256 registers per lane
+// (the two-state backward passes reach ~300) would run one.
 std::string emit(const RegOp* ops, int nops, const Shape& sh) {
     const size_t stride = sizeof(RegOp) / sizeof(double), moff = offsetof(RegOp, m) / sizeof(double);
     const char* arrs[2] = {"a", "l"};
@@ -95,10 +204,10 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
     Src o;
     o.s.reserve(4096 + size_t(nops) * 600 * na);
     if (!sh.batched) {
-        o.line("extern \"C\" __global__ __launch_bounds__(256) void LLSV_KERNEL(v2d* __restrict__ psi, PassDesc d, "
+        o.line("extern \"C\" __global__ __launch_bounds__(256, 2) void LLSV_KERNEL(v2d* __restrict__ psi, PassDesc d, "
                "const double* __restrict__ M) {");
     } else {
-        o.line("extern \"C\" __global__ __launch_bounds__(256) void LLSV_KERNEL(v2d* __restrict__ A, v2d* __restrict__ Lm, "
+        o.line("extern \"C\" __global__ __launch_bounds__(256, 2) void LLSV_KERNEL(v2d* __restrict__ A, v2d* __restrict__ Lm, "
                "unsigned long long size, PassDesc d, const double* __restrict__ T, unsigned long long tstride, "
                "double* __restrict__ partials, int nslot, int ntiles) {");
         o.line("v2d* const psi = A + (unsigned long long)blockIdx.y * size;");
@@ -122,7 +231,8 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
     o.line("for (int j = 0; j < 16; ++j) { const unsigned long long o = base | gt | ((j & 1) ? lg[8] : 0ull) | "
            "((j & 2) ? lg[9] : 0ull) | ((j & 4) ? lg[10] : 0ull) | ((j & 8) ? lg[11] : 0ull);");
     o.line("a[j] = __builtin_nontemporal_load(psi + o);%s } }", sh.two ? " l[j] = __builtin_nontemporal_load(lam + o);" : "");
-    o.line("unsigned int lt = tid, sl = r_swz(tid) << 4;");
+    const Swz sw = choose_swizzle(ops, nops);
+    o.line("unsigned int lt = tid, sl = r_swzm(tid, %uu, %uu, %uu, %uu) << 4;", sw.row[0], sw.row[1], sw.row[2], sw.row[3]);
     o.line("double sr = 1.0, si = 0.0;");
 
     uint32_t S = 0xF00u;  // register bits of the current layout (the kernel starts with 8..11)
@@ -267,12 +377,16 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
                 // one state at a time through the 64 KiB exchange buffer
                 o.line("{ const unsigned int nl = r_ins0(r_ins0(r_ins0(r_ins0(tid, %d), %d), %d), %d);", op.q[0], op.q[1],
                        op.q[2], op.q[3]);
-                o.line("const unsigned int nsl = r_swz(nl) << 4;");
+                o.line("const unsigned int nsl = r_swzm(nl, %uu, %uu, %uu, %uu) << 4;", sw.row[0], sw.row[1], sw.row[2],
+                       sw.row[3]);
+                const uint32_t Snew = (1u << op.q[0]) | (1u << op.q[1]) | (1u << op.q[2]) | (1u << op.q[3]);
                 for (int k = 0; k < na; ++k) {
                     if (k) o.line("__syncthreads();");
-                    for (int j = 0; j < 16; ++j) o.line("*(v2d*)(lds + (sl ^ %uu)) = %s[%d];", op.off0[j], arrs[k], j);
+                    for (int j = 0; j < 16; ++j)
+                        o.line("*(v2d*)(lds + (sl ^ %uu)) = %s[%d];", swz_apply(sw, reg_bits(S, j)) << 4, arrs[k], j);
                     o.line("__syncthreads();");
-                    for (int j = 0; j < 16; ++j) o.line("%s[%d] = *(const v2d*)(lds + (nsl ^ %uu));", arrs[k], j, op.off1[j]);
+                    for (int j = 0; j < 16; ++j)
+                        o.line("%s[%d] = *(const v2d*)(lds + (nsl ^ %uu));", arrs[k], j, swz_apply(sw, reg_bits(Snew, j)) << 4);
                 }
                 o.line("lt = nl; sl = nsl;");
                 o.line("__syncthreads(); }");
