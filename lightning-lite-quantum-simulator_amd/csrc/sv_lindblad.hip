@@ -45,49 +45,58 @@ constexpr int kBM = 64, kBN = 64, kBK = 16;
 
 // GATHER: row r of A is row idx[r0 + r] of the A array when r0 + r < *count, zero otherwise, and
 // workgroups whose first row is at or past *count return at once (the count lives on the device).
-template <bool GATHER>
+// BM: rows of C per workgroup, 64 (2 x 2 wavefronts of 32 x 32) or 16 (1 x 4 wavefronts of one
+// 16 x 16 block each: few rows, as in the gathered jump products).
+template <bool GATHER, int BM>
 __global__ __launch_bounds__(256) void k_zgemm(const v2d* __restrict__ A, int lda, const v2d* __restrict__ B,
                                                int ldb, v2d* __restrict__ C, int ldc, int K,
                                                const uint32_t* __restrict__ idx, const uint32_t* __restrict__ count,
                                                uint32_t r0) {
-    __shared__ double as_r[kBK][kBM], as_i[kBK][kBM], bs_r[kBK][kBN], bs_i[kBK][kBN];
+    static_assert(BM == 64 || BM == 16, "tile rows");
+    constexpr int WN = BM == 64 ? 2 : 4;         // wavefronts along N
+    constexpr int MB = BM == 64 ? 2 : 1;         // 16-row blocks per wavefront
+    constexpr int NB = BM == 64 ? 2 : 1;         // 16-column blocks per wavefront
+    constexpr int EA = BM * kBK / 256;           // A elements per thread per tile (4 or 1)
+    __shared__ double as_r[kBK][BM], as_i[kBK][BM], bs_r[kBK][kBN], bs_i[kBK][kBN];
     const int tid = int(threadIdx.x), lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
-    const int m0 = int(blockIdx.y) * kBM, n0 = int(blockIdx.x) * kBN;
+    const int wm = wave / WN, wn = wave % WN;
+    const int m0 = int(blockIdx.y) * BM, n0 = int(blockIdx.x) * kBN;
     uint32_t rows = 0;
     if (GATHER) {
         const uint32_t cnt = *count;
         if (r0 + uint32_t(m0) >= cnt) return;  // uniform across the workgroup
-        rows = cnt - r0 - uint32_t(m0);        // valid rows of this block (may exceed 64)
+        rows = cnt - r0 - uint32_t(m0);        // valid rows of this block (may exceed BM)
     }
-    d4 cr[2][2], ci[2][2];
-    for (int a = 0; a < 2; ++a)
-        for (int b = 0; b < 2; ++b) {
+    d4 cr[MB][NB], ci[MB][NB];
+    for (int a = 0; a < MB; ++a)
+        for (int b = 0; b < NB; ++b) {
             cr[a][b] = d4{0.0, 0.0, 0.0, 0.0};
             ci[a][b] = d4{0.0, 0.0, 0.0, 0.0};
         }
-    // tile loads: A row ar = tid / 4, columns 4 (tid % 4) .. +3; B row br = tid / 16, columns 4 (tid % 16) .. +3
-    const int ar = tid >> 2, ac = (tid & 3) * 4, br = tid >> 4, bc = (tid & 15) * 4;
+    // tile loads: A row ar, columns ac .. ac + EA - 1; B row br = tid / 16, columns 4 (tid % 16) .. +3
+    const int ar = tid / (kBK / EA), ac = (tid % (kBK / EA)) * EA, br = tid >> 4, bc = (tid & 15) * 4;
     const bool a_live = !GATHER || uint32_t(ar) < rows;
     const v2d* arow = GATHER ? (a_live ? A + size_t(idx[r0 + uint32_t(m0 + ar)]) * lda : A) : A + size_t(m0 + ar) * lda;
     // software pipeline: the next tile's global loads are in flight while this tile is multiplied
-    v2d va[4], vb[4];
+    v2d va[EA], vb[4];
     auto load = [&](int k0) {
         const v2d* pa = arow + k0 + ac;
         const v2d* pb = B + size_t(k0 + br) * ldb + n0 + bc;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            va[j] = a_live ? pa[j] : v2d{0.0, 0.0};
-            vb[j] = pb[j];
-        }
+        for (int j = 0; j < EA; ++j) va[j] = a_live ? pa[j] : v2d{0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) vb[j] = pb[j];
     };
     load(0);
     for (int k0 = 0; k0 < K; k0 += kBK) {
         __syncthreads();  // the previous tile is no longer read
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < EA; ++j) {
             as_r[ac + j][ar] = va[j].x;
             as_i[ac + j][ar] = va[j].y;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
             bs_r[br][bc + j] = vb[j].x;
             bs_i[br][bc + j] = vb[j].y;
         }
@@ -96,29 +105,29 @@ __global__ __launch_bounds__(256) void k_zgemm(const v2d* __restrict__ A, int ld
 #pragma unroll
         for (int ks = 0; ks < kBK / 4; ++ks) {
             const int kk = ks * 4 + (lane >> 4);
-            double a_r[2], a_i[2], a_n[2], b_r[2], b_i[2];
+            double a_r[MB], a_i[MB], a_n[MB], b_r[NB], b_i[NB];
 #pragma unroll
-            for (int mb = 0; mb < 2; ++mb) {
-                a_r[mb] = as_r[kk][wm * 32 + mb * 16 + (lane & 15)];
-                a_i[mb] = as_i[kk][wm * 32 + mb * 16 + (lane & 15)];
+            for (int mb = 0; mb < MB; ++mb) {
+                a_r[mb] = as_r[kk][wm * MB * 16 + mb * 16 + (lane & 15)];
+                a_i[mb] = as_i[kk][wm * MB * 16 + mb * 16 + (lane & 15)];
                 a_n[mb] = -a_i[mb];
             }
 #pragma unroll
-            for (int nb = 0; nb < 2; ++nb) {
-                b_r[nb] = bs_r[kk][wn * 32 + nb * 16 + (lane & 15)];
-                b_i[nb] = bs_i[kk][wn * 32 + nb * 16 + (lane & 15)];
+            for (int nb = 0; nb < NB; ++nb) {
+                b_r[nb] = bs_r[kk][wn * NB * 16 + nb * 16 + (lane & 15)];
+                b_i[nb] = bs_i[kk][wn * NB * 16 + nb * 16 + (lane & 15)];
             }
 #pragma unroll
-            for (int mb = 0; mb < 2; ++mb)
+            for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-                for (int nb = 0; nb < 2; ++nb) {
+                for (int nb = 0; nb < NB; ++nb) {
                     cr[mb][nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[mb], b_r[nb], cr[mb][nb], 0, 0, 0);
                     ci[mb][nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_r[mb], b_i[nb], ci[mb][nb], 0, 0, 0);
                 }
 #pragma unroll
-            for (int mb = 0; mb < 2; ++mb)
+            for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-                for (int nb = 0; nb < 2; ++nb) {
+                for (int nb = 0; nb < NB; ++nb) {
                     cr[mb][nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_n[mb], b_i[nb], cr[mb][nb], 0, 0, 0);
                     ci[mb][nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_i[mb], b_r[nb], ci[mb][nb], 0, 0, 0);
                 }
@@ -126,13 +135,13 @@ __global__ __launch_bounds__(256) void k_zgemm(const v2d* __restrict__ A, int ld
     }
     // D[i][j] of a 16 x 16 block: lane j + 16 (i mod 4), element i / 4
 #pragma unroll
-    for (int mb = 0; mb < 2; ++mb)
+    for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-        for (int nb = 0; nb < 2; ++nb)
+        for (int nb = 0; nb < NB; ++nb)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = m0 + wm * 32 + mb * 16 + 4 * v + (lane >> 4);
-                const int col = n0 + wn * 32 + nb * 16 + (lane & 15);
+                const int row = m0 + wm * MB * 16 + mb * 16 + 4 * v + (lane >> 4);
+                const int col = n0 + wn * NB * 16 + nb * 16 + (lane & 15);
                 v2d y;
                 y.x = cr[mb][nb][v];
                 y.y = ci[mb][nb][v];
@@ -399,16 +408,16 @@ __global__ __launch_bounds__(256) void k_diffuse(v2d* __restrict__ psi, const v2
 int pad_dim(int dim) { return dim < kBN ? kBN : dim; }
 
 hipError_t gemm(hipStream_t st, const v2d* A, int lda, const v2d* B, int ldb, v2d* C, int ldc, int M, int N, int K) {
-    hipLaunchKernelGGL(k_zgemm<false>, dim3(unsigned(N / kBN), unsigned(M / kBM)), dim3(256), 0, st, A, lda, B, ldb, C,
-                       ldc, K, nullptr, nullptr, 0u);
+    hipLaunchKernelGGL((k_zgemm<false, 64>), dim3(unsigned(N / kBN), unsigned(M / kBM)), dim3(256), 0, st, A, lda, B, ldb,
+                       C, ldc, K, nullptr, nullptr, 0u);
     return hipGetLastError();
 }
 
 // C = A[idx[r0 ..]] B over at most M gathered rows; blocks past the device-side count return.
 hipError_t gemm_gather(hipStream_t st, const v2d* A, int lda, const uint32_t* idx, const uint32_t* count, uint32_t r0,
                        const v2d* B, int ldb, v2d* C, int ldc, int M, int N, int K) {
-    hipLaunchKernelGGL(k_zgemm<true>, dim3(unsigned(N / kBN), unsigned(M / kBM)), dim3(256), 0, st, A, lda, B, ldb, C,
-                       ldc, K, idx, count, r0);
+    hipLaunchKernelGGL((k_zgemm<true, 16>), dim3(unsigned(N / kBN), unsigned(M / 16)), dim3(256), 0, st, A, lda, B, ldb,
+                       C, ldc, K, idx, count, r0);
     return hipGetLastError();
 }
 
@@ -437,19 +446,38 @@ std::vector<double> padded(const double* m, int dim, int D, bool transpose) {
 
 struct DevBuf {
     void* p = nullptr;
+    size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() {
         if (p) (void)hipFree(p);
     }
     hipError_t alloc(size_t bytes) {
         if (p) (void)hipFree(p);
         p = nullptr;
-        return hipMalloc(&p, bytes ? bytes : 16);
+        cap = 0;
+        const hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
+        if (e == hipSuccess) cap = bytes ? bytes : 16;
+        return e;
+    }
+    // grow-only: keeps the allocation when it is large enough
+    hipError_t need(size_t bytes) { return bytes <= cap && p ? hipSuccess : alloc(bytes); }
+    void swap(DevBuf& o) {
+        std::swap(p, o.p);
+        std::swap(cap, o.cap);
     }
     template <class T>
     T* as() const {
         return static_cast<T*>(p);
     }
 };
+
+// Rows 0 .. T-1 of X (Tp rows of D) = src, rows T .. Tp-1 = 0.
+__global__ void k_fill_rows(v2d* __restrict__ X, const v2d* __restrict__ src, uint64_t T, uint64_t Tp, int D) {
+    for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < Tp * uint64_t(D); i += uint64_t(gridDim.x) * 256)
+        X[i] = (i / D < T) ? src[i % D] : v2d{0.0, 0.0};
+}
 
 }  // namespace
 }  // namespace llsv
@@ -461,6 +489,8 @@ struct llsv_lplan {
     llsv::DevBuf nodes;  // propagator or RK4 / drift nodes, transposed and padded (D x D each)
     llsv::DevBuf ops;    // jump operators (padded, row-major), or for ANALOG the stacked
                          // [C_1^T .. C_n^T | D_1^T .. D_n^T] (D x 2nD)
+    // working buffers of llsv_lplan_run, kept (grow-only) across runs of the plan
+    llsv::DevBuf psi, tmp, cur, surv, flag, dtab, psi0, w1, k1, k2, k3, k4, cpb, jl, jc, ybuf, at, bc, rs;
 };
 
 extern "C" {
@@ -500,13 +530,13 @@ int llsv_expm(int dim, const double* a, double* out) {
     for (int k = 1; k <= 18; ++k) {
         XK(gemm(st, term.as<v2d>(), D, b.as<v2d>(), D, tmp.as<v2d>(), D, D, D, D));
         hipLaunchKernelGGL(k_div, dim3(grid_of(nn)), dim3(256), 0, st, tmp.as<v2d>(), nn, double(k));
-        std::swap(term.p, tmp.p);
+        term.swap(tmp);
         hipLaunchKernelGGL(k_add, dim3(grid_of(nn)), dim3(256), 0, st, res.as<v2d>(), term.as<const v2d>(), nn);
         XK(hipGetLastError());
     }
     for (int i = 0; i < s; ++i) {
         XK(gemm(st, res.as<v2d>(), D, res.as<v2d>(), D, tmp.as<v2d>(), D, D, D, D));
-        std::swap(res.p, tmp.p);
+        res.swap(tmp);
     }
     std::vector<double> hr(2 * nn);
     XK(hipMemcpy(hr.data(), res.p, nn * 16, hipMemcpyDeviceToHost));
@@ -690,21 +720,27 @@ int llsv_lplan_run(llsv_lplan* p, const double* psi0, int64_t n_traj, int64_t fi
         for (auto& th : pool) th.join();
     }
     const size_t sbytes = size_t(Tp) * D * 16;
-    DevBuf psi, tmp, cur, surv, flag, dtab, w1, w2, k1, k2, k3, k4, cpb, jl, jc, ybuf;
-    RK(psi.alloc(sbytes));
-    RK(tmp.alloc(sbytes));
-    RK(cur.alloc(sizeof(uint64_t) * T));
-    RK(surv.alloc(sizeof(double) * T));
-    RK(flag.alloc(sizeof(int)));
-    RK(hipMemset(psi.p, 0, sbytes));
-    RK(hipMemset(tmp.p, 0, sbytes));
+    DevBuf &psi = p->psi, &tmp = p->tmp, &cur = p->cur, &surv = p->surv, &flag = p->flag, &dtab = p->dtab,
+           &w1 = p->w1, &k1 = p->k1, &k2 = p->k2, &k3 = p->k3, &k4 = p->k4, &cpb = p->cpb, &jl = p->jl, &jc = p->jc,
+           &ybuf = p->ybuf;
+    RK(psi.need(sbytes));
+    RK(tmp.need(sbytes));
+    RK(cur.need(sizeof(uint64_t) * T));
+    RK(surv.need(sizeof(double) * T));
+    RK(flag.need(sizeof(int)));
+    RK(p->psi0.need(size_t(D) * 16));
     RK(hipMemset(cur.p, 0, sizeof(uint64_t) * T));
     RK(hipMemset(flag.p, 0, sizeof(int)));
-    std::vector<double> hs(size_t(2) * D * T, 0.0);  // trajectories as rows of D amplitudes
-    for (uint64_t t = 0; t < T; ++t) std::memcpy(hs.data() + 2 * t * D, psi0, sizeof(double) * 2 * dim);
-    RK(hipMemcpy(psi.p, hs.data(), hs.size() * sizeof(double), hipMemcpyHostToDevice));
+    {
+        std::vector<double> h0(size_t(2) * D, 0.0);  // psi0 padded to D amplitudes, broadcast on the device
+        std::memcpy(h0.data(), psi0, sizeof(double) * 2 * dim);
+        RK(hipMemcpy(p->psi0.p, h0.data(), h0.size() * sizeof(double), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_fill_rows, dim3(grid_of(Tp * D)), dim3(256), 0, st, psi.as<v2d>(), p->psi0.as<const v2d>(), T,
+                           Tp, D);
+        RK(hipGetLastError());
+    }
     if (!table.empty()) {
-        RK(dtab.alloc(table.size() * sizeof(double)));
+        RK(dtab.need(table.size() * sizeof(double)));
         RK(hipMemcpy(dtab.p, table.data(), table.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     Draws dr{table.empty() ? nullptr : dtab.as<const double>(), stride, seed, uint64_t(first_traj)};
@@ -712,26 +748,26 @@ int llsv_lplan_run(llsv_lplan* p, const double* psi0, int64_t n_traj, int64_t fi
     // rounds of at most R rows (Y of R x nD entries, at most 512 MiB)
     uint32_t R = 0;
     if (p->kind == 0 && nch > 0) {
-        RK(jl.alloc(sizeof(uint32_t) * T));
-        RK(jc.alloc(sizeof(uint32_t)));
+        RK(jl.need(sizeof(uint32_t) * T));
+        RK(jc.need(sizeof(uint32_t)));
         const uint64_t row_bytes = uint64_t(nch) * D * 16;
         R = uint32_t(std::min<uint64_t>(Tp, std::max<uint64_t>(kBM, ((uint64_t(512) << 20) / row_bytes) / kBM * kBM)));
-        RK(ybuf.alloc(size_t(R) * row_bytes));
+        RK(ybuf.need(size_t(R) * row_bytes));
     }
     const uint64_t nn = Tp * D;
     const size_t blk = size_t(D) * D;
     const v2d* nodes = p->nodes.as<const v2d>();
     const bool constant = p->n_nodes == 1;
     if (p->kind == 0 && !constant) {
-        RK(w1.alloc(sbytes));
-        RK(k1.alloc(sbytes));
-        RK(k2.alloc(sbytes));
-        RK(k3.alloc(sbytes));
-        RK(k4.alloc(sbytes));
+        RK(w1.need(sbytes));
+        RK(k1.need(sbytes));
+        RK(k2.need(sbytes));
+        RK(k3.need(sbytes));
+        RK(k4.need(sbytes));
     }
     if (p->kind == 1) {
-        RK(cpb.alloc(size_t(Tp) * std::max(1, 2 * nch) * D * 16));
-        RK(w1.alloc(sbytes));
+        RK(cpb.need(size_t(Tp) * std::max(1, 2 * nch) * D * 16));
+        RK(w1.need(sbytes));
     }
     const double h = p->h;
     for (int64_t s = 0; s < S; ++s) {
@@ -773,7 +809,7 @@ int llsv_lplan_run(llsv_lplan* p, const double* psi0, int64_t n_traj, int64_t fi
             if (constant) {
                 // exact Hamiltonian step, zero drift (src/TrajectorySolver.h:437-441)
                 RK(gemm(st, psi.as<v2d>(), D, nodes, D, tmp.as<v2d>(), D, int(Tp), D, D));
-                std::swap(psi.p, tmp.p);
+                psi.swap(tmp);
             } else {
                 RK(gemm(st, psi.as<v2d>(), D, nodes + blk * size_t(s), D, w1.as<v2d>(), D, int(Tp), D, D));
                 drift = w1.as<const v2d>();
@@ -791,15 +827,16 @@ int llsv_lplan_run(llsv_lplan* p, const double* psi0, int64_t n_traj, int64_t fi
     if (hflag & 1) return fail(LLSV_ERR_NUMERIC, "TrajectorySolver: state norm vanished; reduce dt");
     if (hflag & 2) return fail(LLSV_ERR_NUMERIC, "TrajectorySolver: diffusion state diverged; reduce dt");
     if (states) {
+        std::vector<double> hs(size_t(2) * D * T);  // trajectories as rows of D amplitudes
         RK(hipMemcpy(hs.data(), psi.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
         for (uint64_t t = 0; t < T; ++t) std::memcpy(states + 2 * t * dim, hs.data() + 2 * t * D, sizeof(double) * 2 * dim);
     }
     if (rho_sum) {
         // sum_t psi_t psi_t^+ = Psi^T conj(Psi) (trajectories as rows; padded rows are zero)
-        DevBuf at, bc, rs;
-        RK(at.alloc(sbytes));
-        RK(bc.alloc(sbytes));
-        RK(rs.alloc(size_t(D) * D * 16));
+        DevBuf &at = p->at, &bc = p->bc, &rs = p->rs;
+        RK(at.need(sbytes));
+        RK(bc.need(sbytes));
+        RK(rs.need(size_t(D) * D * 16));
         hipLaunchKernelGGL(k_transpose_conj, dim3(unsigned((Tp * D + 255) / 256)), dim3(256), 0, st, psi.as<const v2d>(),
                            at.as<v2d>(), bc.as<v2d>(), Tp, D);
         RK(hipGetLastError());
