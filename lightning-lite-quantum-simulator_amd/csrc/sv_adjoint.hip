@@ -218,6 +218,129 @@ __global__ __launch_bounds__(256) void k_pc_hpsi(const v2d* __restrict__ A, v2d*
     if (threadIdx.x == 0) partials[size_t(b) * gridDim.x + blockIdx.x] = s;
 }
 
+// ---- H psi over 12-qubit tiles -----------------------------------------------------------
+// The terms are grouped into passes whose X masks fit one tile (qubits 0..2 plus at most nine
+// more); a workgroup loads its tile of psi into LDS once and forms every term of the pass from
+// there, so psi is read once per pass instead of once per term.  Pass contributions add:
+// lambda += H_pass psi (stored in the first pass, accumulated after), and
+// partials[b][pass][tile] = Re <psi|H_pass psi> over the tile.
+struct HTile {
+    int tq[12];  // tile qubits, ascending: local bit k <-> qubit tq[k] (tq[0..2] = 0, 1, 2)
+};
+struct HTerm {
+    double cr, ci;  // c i^ny, ny = popcount(x & z) mod 4 (the Y factors)
+    uint64_t x, z;
+    uint32_t xl;  // x as local bits
+    uint32_t pad;
+};
+
+template <bool STORE>
+__global__ __launch_bounds__(256) void k_pc_hpsi_tile(const v2d* __restrict__ A, v2d* __restrict__ Lm, uint64_t size,
+                                                      HTile tl, const HTerm* __restrict__ terms, int nterms, int first,
+                                                      double* __restrict__ partials, int pass, int npass) {
+    __shared__ v2d tile[4096];
+    __shared__ double lds4[4];
+    const int b = blockIdx.y;
+    const uint32_t tid = threadIdx.x;
+    const v2d* psi = A + uint64_t(b) * size;
+    v2d* lam = STORE ? Lm + uint64_t(b) * size : nullptr;
+    uint64_t base = blockIdx.x;
+    for (int j = 0; j < 12; ++j) base = insert0(base, tl.tq[j]);
+    uint32_t glane = 0, greg[16];  // states here have at most 30 qubits: 32-bit indices
+    for (int k = 0; k < 8; ++k)
+        if ((tid >> k) & 1u) glane |= 1u << tl.tq[k];
+    v2d own[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        uint32_t gr = uint32_t(base) | glane;
+        for (int k = 0; k < 4; ++k)
+            if ((r >> k) & 1) gr |= 1u << tl.tq[8 + k];
+        greg[r] = gr;
+        own[r] = psi[gr];
+        tile[tid + 256u * uint32_t(r)] = own[r];
+    }
+    __syncthreads();
+    double lr[16], li[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) lr[r] = li[r] = 0.0;
+    // one term at a time over the thread's 16 amplitudes, branch-free (no control flow merges the
+    // accumulators): (P psi)_g = c i^ny (-1)^popc(src & z) psi_src, src = g ^ x
+    for (int t = 0; t < nterms; ++t) {
+        const HTerm T = terms[first + t];
+        const uint32_t x = uint32_t(T.x), z = uint32_t(T.z), xl = T.xl;
+        const double cr = T.cr, ci = T.ci;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const v2d a = tile[(tid + 256u * uint32_t(r)) ^ xl];
+            const bool neg = __popc((greg[r] ^ x) & z) & 1;
+            const double sr = neg ? -cr : cr, si = neg ? -ci : ci;
+            lr[r] = __builtin_fma(sr, a.x, __builtin_fma(-si, a.y, lr[r]));
+            li[r] = __builtin_fma(sr, a.y, __builtin_fma(si, a.x, li[r]));
+        }
+    }
+    double e = 0.0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        e += own[r].x * lr[r] + own[r].y * li[r];
+        if (STORE) {
+            v2d o;
+            o.x = lr[r];
+            o.y = li[r];
+            if (pass > 0) o = o + lam[greg[r]];
+            lam[greg[r]] = o;
+        }
+    }
+    const double sum = bsum(e, lds4);
+    if (tid == 0) partials[(size_t(b) * npass + pass) * gridDim.x + blockIdx.x] = sum;
+}
+
+// Groups the terms into at most four tile passes (diagonal terms in the first); false when some
+// X mask cannot be placed (the per-term gather kernel runs instead).
+bool plan_hpsi(int n, const llsv_pauli* terms, int T, std::vector<HTile>& tiles, std::vector<int>& first,
+               std::vector<int>& count, std::vector<HTerm>& out) {
+    if (n < 12) return false;
+    std::vector<char> done(size_t(T), 0);
+    int left = T;
+    while (left > 0) {
+        if (tiles.size() == 4) return false;
+        uint32_t bits = 0x7u;  // qubits 0..2
+        const size_t start = out.size();
+        for (int t = 0; t < T; ++t) {
+            if (done[t]) continue;
+            const uint64_t x = terms[t].xmask;
+            if (x >> 32) return false;
+            const uint32_t nb = bits | uint32_t(x);
+            if (__builtin_popcount(nb) > 12) continue;
+            bits = nb;
+        }
+        for (int q = 0; q < n && __builtin_popcount(bits) < 12; ++q) bits |= 1u << q;
+        HTile tl{};
+        int k = 0;
+        for (int q = 0; q < n; ++q)
+            if ((bits >> q) & 1u) tl.tq[k++] = q;
+        for (int t = 0; t < T; ++t) {
+            if (done[t] || (terms[t].xmask & ~uint64_t(bits))) continue;
+            HTerm h{};
+            h.x = terms[t].xmask;
+            h.z = terms[t].zmask;
+            for (int j = 0; j < 12; ++j)
+                if ((h.x >> tl.tq[j]) & 1u) h.xl |= 1u << j;
+            const int ny = __builtin_popcountll(h.x & h.z) & 3;
+            const double c = terms[t].coeff;
+            h.cr = ny == 0 ? c : ny == 2 ? -c : 0.0;  // c i^ny
+            h.ci = ny == 1 ? c : ny == 3 ? -c : 0.0;
+            out.push_back(h);
+            done[t] = 1;
+            --left;
+        }
+        if (out.size() == start) return false;
+        tiles.push_back(tl);
+        first.push_back(int(start));
+        count.push_back(int(out.size() - start));
+    }
+    return true;
+}
+
 // out[b * nslot + s] = sum over nblk partials (fixed order); grid (nslot, B).
 __global__ __launch_bounds__(256) void k_pc_reduce(const double* __restrict__ partials, int nblk, int nslot,
                                                    double* __restrict__ out) {
@@ -258,7 +381,7 @@ Layout layout_of(int n, int B, int n_gates, int nslot, int nterms) {
     const uint64_t per_slot = std::max<uint64_t>(kRedGrid, tiled_tiles(n));
     L.partials = al(sizeof(double) * size_t(B) * (size_t(n_gates) + 1) * per_slot);
     L.slots = al(sizeof(double) * size_t(B) * (size_t(n_gates) + 1));
-    L.terms = al(sizeof(PauliDev) * size_t(nterms > 0 ? nterms : 1));
+    L.terms = al(std::max(sizeof(PauliDev), sizeof(HTerm)) * size_t(nterms > 0 ? nterms : 1));
     L.slotmap = al(sizeof(int) * size_t(n_gates + 1));
     L.table = al(sizeof(double) * 16 * size_t(B) * size_t(n_gates + 1));  // forward + inverse 2x2 per one-qubit gate
     L.total = 2 * L.states + L.partials + L.slots + L.terms + L.slotmap + L.table;
@@ -561,14 +684,36 @@ int run(int n, int B, const llsv_pgate* gates, int G, int P, const double* param
         for (int k = 0; k < G; ++k) CK(apply_gate(st, A, nullptr, false, L, B, gates[k], params, P, false, partials, -1));
     }
 
-    const dim3 rgrid(unsigned(L.nblk), unsigned(B));
-    if (adj)
-        hipLaunchKernelGGL(k_pc_hpsi<true>, rgrid, dim3(256), 0, st, A, Lm, L.size, dterms, T, partials);
-    else
-        hipLaunchKernelGGL(k_pc_hpsi<false>, rgrid, dim3(256), 0, st, A, Lm, L.size, dterms, T, partials);
-    CK(hipGetLastError());
-    hipLaunchKernelGGL(k_pc_reduce, dim3(1, unsigned(B)), dim3(256), 0, st, partials, L.nblk, 1, energies);
-    CK(hipGetLastError());
+    std::vector<HTile> htiles;
+    std::vector<int> hfirst, hcount;
+    std::vector<HTerm> hterms;
+    // (the partials area holds B x (G + 1) x ntiles values: at most G + 1 passes)
+    if (tiled && plan_hpsi(n, terms, T, htiles, hfirst, hcount, hterms) && int(htiles.size()) <= G + 1) {
+        // lambda = H psi and the energies, tile pass by tile pass
+        HTerm* dht = reinterpret_cast<HTerm*>(dterms);
+        CK(hipMemcpyAsync(dht, hterms.data(), sizeof(HTerm) * hterms.size(), hipMemcpyHostToDevice, st));
+        const int np = int(htiles.size());
+        for (int ps = 0; ps < np; ++ps) {
+            if (adj)
+                hipLaunchKernelGGL(k_pc_hpsi_tile<true>, dim3(unsigned(ntiles), unsigned(B)), dim3(256), 0, st, A, Lm,
+                                   L.size, htiles[ps], dht, hcount[ps], hfirst[ps], partials, ps, np);
+            else
+                hipLaunchKernelGGL(k_pc_hpsi_tile<false>, dim3(unsigned(ntiles), unsigned(B)), dim3(256), 0, st, A, Lm,
+                                   L.size, htiles[ps], dht, hcount[ps], hfirst[ps], partials, ps, np);
+            CK(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_pc_reduce, dim3(1, unsigned(B)), dim3(256), 0, st, partials, np * ntiles, 1, energies);
+        CK(hipGetLastError());
+    } else {
+        const dim3 rgrid(unsigned(L.nblk), unsigned(B));
+        if (adj)
+            hipLaunchKernelGGL(k_pc_hpsi<true>, rgrid, dim3(256), 0, st, A, Lm, L.size, dterms, T, partials);
+        else
+            hipLaunchKernelGGL(k_pc_hpsi<false>, rgrid, dim3(256), 0, st, A, Lm, L.size, dterms, T, partials);
+        CK(hipGetLastError());
+        hipLaunchKernelGGL(k_pc_reduce, dim3(1, unsigned(B)), dim3(256), 0, st, partials, L.nblk, 1, energies);
+        CK(hipGetLastError());
+    }
     if (!adj) return LLSV_OK;
 
     int per_slot = L.nblk;
