@@ -342,6 +342,21 @@ struct ZBuf {
     }
 };
 
+// Pinned host staging, grown on demand and kept for the process.
+struct PinBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t need(size_t bytes) {
+        if (bytes <= cap) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        const hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
+};
+
 }  // namespace
 }  // namespace llsv
 
@@ -397,29 +412,13 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
     const int S = int(apps.size());
     const uint64_t N = uint64_t(n_traj), dim = uint64_t(1) << n;
 
-    // events of the reference stream, trajectory by trajectory
-    std::vector<uint2> ev;
-    std::vector<uint64_t> off;
-    if (z->rng == 0) {
-        off.resize(N + 1);
-        off[0] = 0;
-        for (uint64_t t = 0; t < N; ++t) {
-            if (z->p > 0.0)
-                for (int j = 0; j < S; ++j) {
-                    const uint32_t c = mt_site(z->mt, z->p, apps[size_t(j)].nq);
-                    if (c) ev.push_back(make_uint2(uint32_t(j), c));
-                }
-            off[t + 1] = ev.size();
-        }
-    }
-
     // launch shape: L threads per trajectory, 256 / L trajectories per workgroup
     int lgL = std::min(8, std::max(0, n - 1));
     const int G = kZThreads >> lgL;
     const size_t lds = size_t(G) * dim * 16;
     static std::mutex mu;  // the device buffers below are shared by every handle
     std::lock_guard<std::mutex> lock(mu);
-    static ZBuf b_apps, b_ev, b_off, b_w, b_vals, b_amps;
+    static ZBuf b_apps, b_w, b_vals, b_amps;
     static bool attr_set = false;
     if (!attr_set) {
         // up to 64 KiB of dynamic LDS (one 12-qubit state)
@@ -436,37 +435,67 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
     } while (0)
     ZK(b_apps.need(sizeof(ZApp) * apps.size()));
     ZK(hipMemcpy(b_apps.p, apps.data(), sizeof(ZApp) * apps.size(), hipMemcpyHostToDevice));
-    if (z->rng == 0) {
-        ZK(b_off.need(sizeof(uint64_t) * off.size()));
-        ZK(hipMemcpy(b_off.p, off.data(), sizeof(uint64_t) * off.size(), hipMemcpyHostToDevice));
-        ZK(b_ev.need(sizeof(uint2) * std::max<size_t>(1, ev.size())));
-        if (!ev.empty()) ZK(hipMemcpy(b_ev.p, ev.data(), sizeof(uint2) * ev.size(), hipMemcpyHostToDevice));
-    }
     if (weights) {
         ZK(b_w.need(sizeof(double) * dim));
         ZK(hipMemcpy(b_w.p, weights, sizeof(double) * dim, hipMemcpyHostToDevice));
         ZK(b_vals.need(sizeof(double) * N));
     }
-    // final states go out in chunks of at most 256 MiB
-    const uint64_t chunk = amps ? std::max<uint64_t>(1, (uint64_t(256) << 20) / (dim * 16)) : N;
+    // Trajectories go in chunks: final states at most 256 MiB at a time, and with the reference
+    // stream at most 2048 trajectories, so the host draws chunk c + 1 while the device runs chunk c
+    // (two event buffers, pinned host staging, an event per buffer).
+    uint64_t chunk = N;
+    if (amps) chunk = std::max<uint64_t>(1, (uint64_t(256) << 20) / (dim * 16));
+    if (z->rng == 0) chunk = std::min<uint64_t>(chunk, 2048);
     if (amps) ZK(b_amps.need(size_t(std::min(chunk, N) * dim * 16)));
-    for (uint64_t t0 = 0; t0 < N; t0 += chunk) {
+    static ZBuf b_ev[2], b_off[2];
+    static PinBuf h_ev[2], h_off[2];
+    static hipEvent_t done[2] = {nullptr, nullptr};
+    for (int k = 0; k < 2; ++k)
+        if (!done[k]) ZK(hipEventCreateWithFlags(&done[k], hipEventDisableTiming));
+    std::vector<uint2> ev;
+    std::vector<uint64_t> off;
+    int c = 0;
+    for (uint64_t t0 = 0; t0 < N; t0 += chunk, ++c) {
         const uint64_t nt = std::min(chunk, N - t0);
+        const int slot = c & 1;
+        if (z->rng == 0) {
+            if (c >= 2) ZK(hipEventSynchronize(done[slot]));  // chunk c - 2 no longer reads this slot
+            // the reference stream, trajectory by trajectory, site by site
+            ev.clear();
+            off.assign(nt + 1, 0);
+            for (uint64_t t = 0; t < nt; ++t) {
+                if (z->p > 0.0)
+                    for (int j = 0; j < S; ++j) {
+                        const uint32_t code = mt_site(z->mt, z->p, apps[size_t(j)].nq);
+                        if (code) ev.push_back(make_uint2(uint32_t(j), code));
+                    }
+                off[t + 1] = ev.size();
+            }
+            const size_t evb = sizeof(uint2) * std::max<size_t>(1, ev.size()), offb = sizeof(uint64_t) * off.size();
+            ZK(h_ev[slot].need(evb));
+            ZK(h_off[slot].need(offb));
+            if (!ev.empty()) std::memcpy(h_ev[slot].p, ev.data(), sizeof(uint2) * ev.size());
+            std::memcpy(h_off[slot].p, off.data(), offb);
+            ZK(b_ev[slot].need(evb));
+            ZK(b_off[slot].need(offb));
+            if (!ev.empty()) ZK(hipMemcpyAsync(b_ev[slot].p, h_ev[slot].p, sizeof(uint2) * ev.size(), hipMemcpyHostToDevice, nullptr));
+            ZK(hipMemcpyAsync(b_off[slot].p, h_off[slot].p, offb, hipMemcpyHostToDevice, nullptr));
+        }
         const unsigned grid = unsigned(std::min<uint64_t>((nt + G - 1) / G, 16384));
-        const uint64_t* doff = z->rng == 0 ? static_cast<const uint64_t*>(b_off.p) + t0 : nullptr;
         // w is null unless weights were given (the cached buffer may hold an earlier call's)
         const double* dw = weights ? static_cast<const double*>(b_w.p) : nullptr;
         double* dvals = weights ? static_cast<double*>(b_vals.p) + t0 : nullptr;
         v2d* damps = amps ? static_cast<v2d*>(b_amps.p) : nullptr;
         if (z->rng == 0)
             hipLaunchKernelGGL(k_zne<false>, dim3(grid), dim3(kZThreads), lds, nullptr, static_cast<const ZApp*>(b_apps.p), S,
-                               n, lgL, nt, static_cast<const uint2*>(b_ev.p), doff, z->p, z->seed, 0,
-                               dw, dvals, damps);
+                               n, lgL, nt, static_cast<const uint2*>(b_ev[slot].p),
+                               static_cast<const uint64_t*>(b_off[slot].p), z->p, z->seed, 0, dw, dvals, damps);
         else
             hipLaunchKernelGGL(k_zne<true>, dim3(grid), dim3(kZThreads), lds, nullptr, static_cast<const ZApp*>(b_apps.p), S,
                                n, lgL, nt, nullptr, nullptr, z->p, z->seed, z->philox_next + t0,
                                dw, dvals, damps);
         ZK(hipGetLastError());
+        if (z->rng == 0) ZK(hipEventRecord(done[slot], nullptr));
         if (amps) ZK(hipMemcpy(amps + 2 * t0 * dim, damps, size_t(nt * dim * 16), hipMemcpyDeviceToHost));
     }
     if (weights) ZK(hipMemcpy(values, b_vals.p, sizeof(double) * N, hipMemcpyDeviceToHost));
