@@ -245,7 +245,17 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
             case RO_1Q:
             case RO_1QT: {
                 const int K = 1 << op.a;
-                const int form = op.kind == RO_1QT ? op.q[0] : 0;
+                // RO_1QT carries its form; an RO_1Q matrix of the RX or RY pattern (zeros and equal
+                // entries where the rotation has them) runs the 8-operation form, which rounds exactly
+                // like the general update on such a matrix (the dropped terms are products with zero)
+                int form = op.kind == RO_1QT ? op.q[0] : 0;
+                if (op.kind == RO_1Q) {
+                    const double* m = op.m;
+                    if (m[1] == 0.0 && m[3] == 0.0 && m[5] == 0.0 && m[7] == 0.0 && m[0] == m[6] && m[2] == -m[4])
+                        form = 2;
+                    else if (m[1] == 0.0 && m[2] == 0.0 && m[4] == 0.0 && m[7] == 0.0 && m[0] == m[6] && m[3] == m[5])
+                        form = 1;
+                }
                 const size_t t8 = size_t(8) * size_t(op.moff);
                 if (op.kind == RO_1Q)
                     o.line("{ const double* m = M + %zu;", c);
