@@ -419,6 +419,20 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
     static std::mutex mu;  // the device buffers below are shared by every handle
     std::lock_guard<std::mutex> lock(mu);
     static ZBuf b_apps, b_w, b_vals, b_amps;
+    static ZBuf b_ev[2], b_off[2];
+    static int owner = -1;  // the device the cached buffers live on
+    {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_noisy_run: no device");
+        if (owner != dev) {  // another device is current: the buffers are allocated again there
+            for (ZBuf* b : {&b_apps, &b_w, &b_vals, &b_amps, &b_ev[0], &b_ev[1], &b_off[0], &b_off[1]}) {
+                if (b->p) (void)hipFree(b->p);
+                b->p = nullptr;
+                b->cap = 0;
+            }
+            owner = dev;
+        }
+    }
     static bool attr_set = false;
     if (!attr_set) {
         // up to 64 KiB of dynamic LDS (one 12-qubit state)
@@ -447,7 +461,6 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
     if (amps) chunk = std::max<uint64_t>(1, (uint64_t(256) << 20) / (dim * 16));
     if (z->rng == 0) chunk = std::min<uint64_t>(chunk, 2048);
     if (amps) ZK(b_amps.need(size_t(std::min(chunk, N) * dim * 16)));
-    static ZBuf b_ev[2], b_off[2];
     static PinBuf h_ev[2], h_off[2];
     static hipEvent_t done[2] = {nullptr, nullptr};
     for (int k = 0; k < 2; ++k)
