@@ -158,3 +158,20 @@ def test_norm_vanishes_raises():
     s = solver(1, None, [(SM, 1e6)])
     with pytest.raises(RuntimeError):
         s.run_one(np.array([0, 1.0]), 1.0, 1.0, 1)
+
+
+def test_unitary_evolution_closed_form():
+    # no channels: every unraveling is the exact propagator exp(-i H t) (no draws)
+    from scipy.linalg import expm
+
+    import quantum_sim_trajectory as QT
+
+    H, _ = two_qubit_model()
+    s = solver(2, H, [])
+    psi0 = np.array([0.6, 0.0, 0.8j, 0.0])
+    want = expm(-1j * H * 1.3) @ psi0
+    for kind in (QT.Unraveling.STANDARD, QT.Unraveling.ANALOG):
+        got = s.run_one(psi0, 1.3, 0.01, 7, kind)
+        assert np.max(np.abs(got - want)) < 1e-12
+    with pytest.raises(ValueError):
+        s.estimate_expectation(psi0, 1.0, 0.01, 4, np.kron(SM, np.eye(2)))

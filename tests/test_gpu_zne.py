@@ -211,3 +211,13 @@ def test_errors_and_size_limits():
                               lambda b: float(b.probabilities()[0]), [1, 3])
     o = Z.extrapolate(n, gates, 0.1, 4, 6, [1, 3], np.eye(1, 2 ** n, 0).ravel())
     assert abs(r.value - o["value"]) < 1e-12
+
+
+def test_every_site_errs_and_one_qubit():
+    # p = 1: a Pauli after every application (the reference stream draws a choice at every site)
+    for n, gates, scale in ((1, [("x", 0), ("mcz", [0]), ("ry", 0, 0.4), ("h", 0)], 3),
+                            (3, CIRC3, 1)):
+        w = np.random.default_rng(n).uniform(-1, 1, 2 ** n)
+        ours = gpu_values(n, gates, 1.0, 5, scale, 64, w)
+        theirs = Z.noisy_values(n, gates, 1.0, 5, scale, 64, w)
+        assert np.max(np.abs(ours - theirs)) < 1e-12
