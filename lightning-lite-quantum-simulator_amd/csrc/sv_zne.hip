@@ -6,7 +6,7 @@
 // from the reference's sequential stream (std::mt19937_64, the draws ll::NoisyBackend makes), or
 // from counter-based Philox draws made on the device.  Nothing is normalised and nothing waits on
 // a probability, so a trajectory is a plain gate sweep: each one lives in LDS (2^n x 16 B, n <= 12)
-// and owns L = min(256, 2^(n-1)) threads of a 256-thread workgroup (256 / L trajectories per
+// and owns L = min(512, 2^(n-1)) threads of a 512-thread workgroup (512 / L trajectories per
 // workgroup).  Per gate application: one pass over the gate's amplitude pairs (one-qubit gates),
 // quads (cnot, two-qubit mcz) or single amplitudes (wider mcz); the sampled Pauli acts on the same
 // qubits and is applied in the same pass (X^x Z^z on the pair or quad), a workgroup barrier ends
@@ -28,7 +28,7 @@ namespace llsv {
 
 namespace {
 
-constexpr int kZThreads = 256;
+constexpr int kZThreads = 512;
 
 // One gate application of the folded circuit.
 struct ZApp {
@@ -98,7 +98,7 @@ __device__ __forceinline__ void code_masks(const ZApp& a, uint32_t code, uint32_
 
 __device__ __forceinline__ v2d neg_if(v2d v, bool neg) { return neg ? -v : v; }
 
-// Group-wide sum of L consecutive lanes (L a power of two) of a 256-thread workgroup.
+// Group-wide sum of L consecutive lanes (L a power of two) of a kZThreads-thread workgroup.
 __device__ double group_sum(double v, int L, double* red) {
     const int w = int(threadIdx.x) >> 6;
     for (int o = 1; o < L && o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(kZThreads) void k_zne(const ZApp* __restrict__ apps
                                                    uint64_t traj_base, const double* __restrict__ w,
                                                    double* __restrict__ vals, v2d* __restrict__ amps) {
     extern __shared__ v2d lds_states[];
-    __shared__ double red[4];
+    __shared__ double red[kZThreads / 64];
     const int L = 1 << lgL, G = kZThreads >> lgL;
     const int g = int(threadIdx.x) >> lgL, r = int(threadIdx.x) & (L - 1);
     const uint32_t dim = 1u << n;
@@ -412,8 +412,8 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
     const int S = int(apps.size());
     const uint64_t N = uint64_t(n_traj), dim = uint64_t(1) << n;
 
-    // launch shape: L threads per trajectory, 256 / L trajectories per workgroup
-    int lgL = std::min(8, std::max(0, n - 1));
+    // launch shape: L threads per trajectory, kZThreads / L trajectories per workgroup
+    int lgL = std::min(9, std::max(0, n - 1));
     const int G = kZThreads >> lgL;
     const size_t lds = size_t(G) * dim * 16;
     static std::mutex mu;  // the device buffers below are shared by every handle
