@@ -391,6 +391,15 @@ int llsv_zgemm_time(int M, int N, int K, int reps, float* ms);
 typedef struct llsv_lplan llsv_lplan;
 int llsv_lplan_create(int dim, int kind, int64_t steps, double h, int n_nodes, const double* nodes, int n_ops,
                       const double* ops, llsv_lplan** out);
+
+/* The same plan assembled on the device from the model itself, as TrajectorySolver::make_plan
+ * does on the host (src/TrajectorySolver.h:268-290): H = 1 Hamiltonian (constant: the propagator
+ * exp(-i (H + H_nh) h), scaling and squaring on the device) or 2 steps + 1 (jumps) / `steps`
+ * (diffusive) samples H(t_j); ops = the n_ops channel operators as the unraveling applies them
+ * (sqrt(gamma) L, or sqrt(c gamma) (1 - P) for PROJECTOR); for jumps H_nh = -(i/2) sum C_k^+ C_k
+ * is one GEMM on the device.  Only H and the operators cross PCIe. */
+int llsv_lplan_create_h(int dim, int kind, int64_t steps, double h, int n_h, const double* H, int n_ops,
+                        const double* ops, llsv_lplan** out);
 void llsv_lplan_destroy(llsv_lplan* p);
 
 /* Runs trajectories first_traj .. first_traj + n_traj - 1 of an ensemble from psi0 (dim complex,

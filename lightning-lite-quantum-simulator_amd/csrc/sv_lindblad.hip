@@ -203,6 +203,59 @@ __global__ void k_identity(v2d* __restrict__ X, int D) {
         X[i] = v2d{(i / D == i % D) ? 1.0 : 0.0, 0.0};
 }
 
+// ---- plan assembly on the device ----------------------------------------------------------------
+
+// n row-major dim x dim operators C_k (raw) into BT = [C_1^T | .. | C_n^T] (row stride W, column
+// block k at k D) and CJ = [conj(C_1); ..; conj(C_n)] (n D rows of D).  Padding entries are not
+// written (the caller clears both).
+__global__ void k_stack_ops(const v2d* __restrict__ raw, int n, int dim, int D, int W, v2d* __restrict__ BT,
+                            v2d* __restrict__ CJ) {
+    const uint64_t per = uint64_t(dim) * dim, total = per * uint64_t(n);
+    for (uint64_t e = uint64_t(blockIdx.x) * 256 + threadIdx.x; e < total; e += uint64_t(gridDim.x) * 256) {
+        const uint64_t k = e / per, r = e % per, i = r / dim, j = r % dim;
+        const v2d v = raw[e];
+        BT[j * W + k * D + i] = v;
+        CJ[(k * D + i) * D + j] = v2d{v.x, -v.y};
+    }
+}
+
+// out = (-i s H + g G)^T as a D x D block: out[i][j] = -i s H[j][i] + g GT[i][j] for i, j < dim,
+// zero in the padding.  H row-major dim x dim; GT (D x D) may be null (g G = 0).
+__global__ void k_node_t(const v2d* __restrict__ H, int dim, const v2d* __restrict__ GT, double s, double g,
+                         v2d* __restrict__ out, int D) {
+    const uint64_t nn = uint64_t(D) * D;
+    for (uint64_t e = uint64_t(blockIdx.x) * 256 + threadIdx.x; e < nn; e += uint64_t(gridDim.x) * 256) {
+        const uint64_t i = e / D, j = e % D;
+        v2d v{0.0, 0.0};
+        if (i < uint64_t(dim) && j < uint64_t(dim)) {
+            const v2d x = H[j * dim + i];
+            v = v2d{s * x.y, -(s * x.x)};  // -i s (x + i y)
+            if (GT) v = v + g * GT[e];
+        }
+        out[e] = v;
+    }
+}
+
+// sums[j] = sum_i |X[i][j]|: the column sums of X^T's transpose, i.e. the row sums of the matrix
+// whose transpose X holds (the infinity norm's terms).
+__global__ void k_colabs(const v2d* __restrict__ X, int D, double* __restrict__ sums) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= D) return;
+    double r = 0.0;
+    for (int i = 0; i < D; ++i) {
+        const v2d v = X[size_t(i) * D + j];
+        r += hypot(v.x, v.y);
+    }
+    sums[j] = r;
+}
+
+// Entries of a D x D block outside the leading dim x dim corner = 0.
+__global__ void k_zero_pad(v2d* __restrict__ X, int dim, int D) {
+    const uint64_t nn = uint64_t(D) * D;
+    for (uint64_t e = uint64_t(blockIdx.x) * 256 + threadIdx.x; e < nn; e += uint64_t(gridDim.x) * 256)
+        if (e / D >= uint64_t(dim) || e % D >= uint64_t(dim)) X[e] = v2d{0.0, 0.0};
+}
+
 // ---- random numbers ---------------------------------------------------------------------------
 
 __device__ __host__ inline void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
@@ -473,6 +526,33 @@ struct DevBuf {
     }
 };
 
+// Scaling power of exp(A): the least s >= 0 with ||A||_inf / 2^s <= 1/2 (src/TrajectorySolver.h:97-111).
+int expm_power(double nrm) { return nrm > 0.5 ? std::max(0, int(std::ceil(std::log2(nrm / 0.5)))) : 0; }
+
+// res = exp(A) for the D x D matrix in `a` (scaled in place by 2^-s): 18-term Taylor series of
+// A / 2^s, then s squarings, all GEMMs on the device.  term and tmp are scratch; every buffer holds
+// D x D and the result may end up in any of them but is swapped into `res`.
+hipError_t expm_dev(hipStream_t st, DevBuf& a, DevBuf& res, DevBuf& term, DevBuf& tmp, int D, int s) {
+    const uint64_t nn = uint64_t(D) * D;
+    const unsigned g = grid_of(nn);
+    hipLaunchKernelGGL(k_div, dim3(g), dim3(256), 0, st, a.as<v2d>(), nn, std::ldexp(1.0, s));
+    hipLaunchKernelGGL(k_identity, dim3(g), dim3(256), 0, st, term.as<v2d>(), D);
+    hipLaunchKernelGGL(k_identity, dim3(g), dim3(256), 0, st, res.as<v2d>(), D);
+    hipError_t e = hipGetLastError();
+    for (int k = 1; k <= 18 && e == hipSuccess; ++k) {
+        e = gemm(st, term.as<v2d>(), D, a.as<v2d>(), D, tmp.as<v2d>(), D, D, D, D);
+        hipLaunchKernelGGL(k_div, dim3(g), dim3(256), 0, st, tmp.as<v2d>(), nn, double(k));
+        term.swap(tmp);
+        hipLaunchKernelGGL(k_add, dim3(g), dim3(256), 0, st, res.as<v2d>(), term.as<const v2d>(), nn);
+        if (e == hipSuccess) e = hipGetLastError();
+    }
+    for (int i = 0; i < s && e == hipSuccess; ++i) {
+        e = gemm(st, res.as<v2d>(), D, res.as<v2d>(), D, tmp.as<v2d>(), D, D, D, D);
+        res.swap(tmp);
+    }
+    return e;
+}
+
 // Rows 0 .. T-1 of X (Tp rows of D) = src, rows T .. Tp-1 = 0.
 __global__ void k_fill_rows(v2d* __restrict__ X, const v2d* __restrict__ src, uint64_t T, uint64_t Tp, int D) {
     for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < Tp * uint64_t(D); i += uint64_t(gridDim.x) * 256)
@@ -506,8 +586,8 @@ int llsv_expm(int dim, const double* a, double* out) {
         for (int j = 0; j < dim; ++j) r += std::hypot(a[2 * (size_t(i) * dim + j)], a[2 * (size_t(i) * dim + j) + 1]);
         nrm = std::max(nrm, r);
     }
-    int s = 0;
-    if (nrm > 0.5) s = std::max(0, int(std::ceil(std::log2(nrm / 0.5))));
+    if (!std::isfinite(nrm)) return fail(LLSV_ERR_ARG, "llsv_expm: non-finite matrix");
+    const int s = expm_power(nrm);
     const int D = ((std::max(dim, kBN) + kBN - 1) / kBN) * kBN;
     const uint64_t nn = uint64_t(D) * D;
     DevBuf b, term, res, tmp;
@@ -522,22 +602,7 @@ int llsv_expm(int dim, const double* a, double* out) {
     XK(tmp.alloc(nn * 16));
     const std::vector<double> hb = padded(a, dim, D, false);
     XK(hipMemcpy(b.p, hb.data(), nn * 16, hipMemcpyHostToDevice));
-    const hipStream_t st = nullptr;
-    hipLaunchKernelGGL(k_div, dim3(grid_of(nn)), dim3(256), 0, st, b.as<v2d>(), nn, std::pow(2.0, s));
-    hipLaunchKernelGGL(k_identity, dim3(grid_of(nn)), dim3(256), 0, st, term.as<v2d>(), D);
-    hipLaunchKernelGGL(k_identity, dim3(grid_of(nn)), dim3(256), 0, st, res.as<v2d>(), D);
-    XK(hipGetLastError());
-    for (int k = 1; k <= 18; ++k) {
-        XK(gemm(st, term.as<v2d>(), D, b.as<v2d>(), D, tmp.as<v2d>(), D, D, D, D));
-        hipLaunchKernelGGL(k_div, dim3(grid_of(nn)), dim3(256), 0, st, tmp.as<v2d>(), nn, double(k));
-        term.swap(tmp);
-        hipLaunchKernelGGL(k_add, dim3(grid_of(nn)), dim3(256), 0, st, res.as<v2d>(), term.as<const v2d>(), nn);
-        XK(hipGetLastError());
-    }
-    for (int i = 0; i < s; ++i) {
-        XK(gemm(st, res.as<v2d>(), D, res.as<v2d>(), D, tmp.as<v2d>(), D, D, D, D));
-        res.swap(tmp);
-    }
+    XK(expm_dev(nullptr, b, res, term, tmp, D, s));
     std::vector<double> hr(2 * nn);
     XK(hipMemcpy(hr.data(), res.p, nn * 16, hipMemcpyDeviceToHost));
 #undef XK
@@ -671,6 +736,109 @@ int llsv_lplan_create(int dim, int kind, int64_t steps, double h, int n_nodes, c
             PK(hipDeviceSynchronize());
         }
     }
+#undef PK
+    *out = p.release();
+    return LLSV_OK;
+}
+
+int llsv_lplan_create_h(int dim, int kind, int64_t steps, double h, int n_h, const double* H, int n_ops,
+                        const double* ops, llsv_lplan** out) {
+    using namespace llsv;
+    if (!out) return fail(LLSV_ERR_ARG, "llsv_lplan_create_h: out is null");
+    *out = nullptr;
+    if (dim < 1 || dim > 1024 || (dim & (dim - 1)))
+        return fail(LLSV_ERR_ARG, "llsv_lplan_create_h: dim must be 2^n, n <= 10");
+    if (kind < 0 || kind > 1) return fail(LLSV_ERR_ARG, "llsv_lplan_create_h: kind must be 0 (jumps) or 1 (diffusive)");
+    if (steps < 1 || !(h > 0.0) || !std::isfinite(h))
+        return fail(LLSV_ERR_ARG, "llsv_lplan_create_h: need steps >= 1 and finite h > 0");
+    if (n_ops < 0 || n_ops > 16 || (n_ops && !ops))
+        return fail(LLSV_ERR_ARG, "llsv_lplan_create_h: 0..16 channels");
+    const int64_t want = kind == 0 ? 2 * steps + 1 : steps;
+    if (!H || (n_h != 1 && n_h != want))
+        return fail(LLSV_ERR_ARG, "llsv_lplan_create_h: %d Hamiltonians given, 1 or %lld expected", n_h,
+                    (long long)want);
+    auto p = std::make_unique<llsv_lplan>();
+    p->dim = dim;
+    p->D = pad_dim(dim);
+    p->kind = kind;
+    p->nch = n_ops;
+    p->n_nodes = n_h;
+    p->steps = steps;
+    p->h = h;
+    const int D = p->D, n = n_ops;
+    const uint64_t nn = uint64_t(D) * D;
+    const size_t blk = nn * 16, mat = size_t(dim) * dim * 16;
+    const hipStream_t st = nullptr;
+    const unsigned g = grid_of(nn);
+#define PK(expr)                                                                                      \
+    do {                                                                                              \
+        const hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_lplan_create_h: %s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+    // channels: BT = [C_1^T | .. | C_n^T] (jumps) or [C^T .. | (C^+ C)^T ..] (diffusive), D x W
+    const int W = (kind == 0 ? 1 : 2) * n * D;
+    DevBuf raw, cj, gt;
+    PK(p->ops.alloc(size_t(D) * std::max(W, 1) * 16));
+    if (n > 0) {
+        PK(raw.alloc(mat * n));
+        PK(cj.alloc(blk * n));
+        PK(hipMemcpy(raw.p, ops, mat * n, hipMemcpyHostToDevice));
+        if (dim < D) {
+            PK(hipMemsetAsync(p->ops.p, 0, size_t(D) * W * 16, st));
+            PK(hipMemsetAsync(cj.p, 0, blk * n, st));
+        }
+        hipLaunchKernelGGL(k_stack_ops, dim3(grid_of(uint64_t(dim) * dim * n)), dim3(256), 0, st, raw.as<const v2d>(),
+                           n, dim, D, W, p->ops.as<v2d>(), cj.as<v2d>());
+        PK(hipGetLastError());
+        v2d* stack = p->ops.as<v2d>();
+        if (kind == 1) {
+            // (C_k^+ C_k)^T = C_k^T conj(C_k) into column block n + k
+            for (int k = 0; k < n; ++k)
+                PK(gemm(st, stack + size_t(k) * D, W, cj.as<const v2d>() + size_t(k) * nn, D, stack + size_t(n + k) * D,
+                        W, D, D, D));
+        } else {
+            // G^T = sum_k C_k^T conj(C_k): one GEMM over the stacked channels (K = n D)
+            PK(gt.alloc(blk));
+            PK(gemm(st, stack, W, cj.as<const v2d>(), D, gt.as<v2d>(), D, D, D, n * D));
+        }
+    }
+    // nodes, transposed: the jump kinds add H_nh = -(i/2) G, so -i (H + H_nh) = -i H - G / 2
+    const v2d* GT = (kind == 0 && n > 0) ? gt.as<const v2d>() : nullptr;
+    DevBuf hraw;
+    PK(hraw.alloc(mat));
+    if (n_h == 1) {
+        // propagator exp(-i (H + H_nh) h)^T = exp(A^T), A = -i h H - (h / 2) G
+        DevBuf a, res, term, tmp, sums;
+        PK(a.alloc(blk));
+        PK(res.alloc(blk));
+        PK(term.alloc(blk));
+        PK(tmp.alloc(blk));
+        PK(sums.alloc(size_t(D) * sizeof(double)));
+        PK(hipMemcpy(hraw.p, H, mat, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_node_t, dim3(g), dim3(256), 0, st, hraw.as<const v2d>(), dim, GT, h, -0.5 * h,
+                           a.as<v2d>(), D);
+        hipLaunchKernelGGL(k_colabs, dim3(unsigned((D + 255) / 256)), dim3(256), 0, st, a.as<const v2d>(), D,
+                           sums.as<double>());
+        PK(hipGetLastError());
+        std::vector<double> hs(D);
+        PK(hipMemcpy(hs.data(), sums.p, size_t(D) * sizeof(double), hipMemcpyDeviceToHost));
+        const double nrm = *std::max_element(hs.begin(), hs.end());
+        if (!std::isfinite(nrm)) return fail(LLSV_ERR_ARG, "llsv_lplan_create_h: non-finite H or channels");
+        PK(expm_dev(st, a, res, term, tmp, D, expm_power(nrm)));
+        if (dim < D) hipLaunchKernelGGL(k_zero_pad, dim3(g), dim3(256), 0, st, res.as<v2d>(), dim, D);
+        PK(hipGetLastError());
+        p->nodes.swap(res);
+    } else {
+        // RK4 / Euler nodes -i (H(t_j) + H_nh), transposed
+        PK(p->nodes.alloc(blk * size_t(n_h)));
+        for (int j = 0; j < n_h; ++j) {
+            PK(hipMemcpy(hraw.p, H + 2 * size_t(j) * dim * dim, mat, hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(k_node_t, dim3(g), dim3(256), 0, st, hraw.as<const v2d>(), dim, GT, 1.0, -0.5,
+                               p->nodes.as<v2d>() + size_t(j) * nn, D);
+            PK(hipGetLastError());
+        }
+    }
+    PK(hipDeviceSynchronize());
 #undef PK
     *out = p.release();
     return LLSV_OK;

@@ -64,6 +64,7 @@ SIGNATURES = {
     "llsv_zgemm_time": (_I, [_I, _I, _I, _I, ctypes.POINTER(ctypes.c_float)]),
     "llsv_expm": (_I, [_I, _P, _P]),
     "llsv_lplan_create": (_I, [_I, _I, ctypes.c_int64, ctypes.c_double, _I, _P, _I, _P, ctypes.POINTER(_P)]),
+    "llsv_lplan_create_h": (_I, [_I, _I, ctypes.c_int64, ctypes.c_double, _I, _P, _I, _P, ctypes.POINTER(_P)]),
     "llsv_lplan_destroy": (None, [_P]),
     "llsv_lplan_run": (_I, [_P, _P, ctypes.c_int64, ctypes.c_int64, _U64, _I, _P, _P]),
     "llsv_pc_precompile": (_I, [_I, _P, _I, ctypes.POINTER(_I)]),

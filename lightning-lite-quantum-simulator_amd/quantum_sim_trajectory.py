@@ -7,8 +7,9 @@ callable t -> matrix), Lindblad channels gamma (L rho L^+ - 1/2 {L^+ L, rho}), a
 unravelings -- STANDARD (quantum jumps), PROJECTOR (Pauli-like channels rewritten with
 C = sqrt(g) (1 - P)) and ANALOG (homodyne diffusion).  The plan (jump operators, H_eff and the
 propagator exp(-i H_eff h), or the sampled H(t) nodes) is built as in src/TrajectorySolver.h:
-268-290; the trajectories then run together on the GPU (llsv_lplan_run): every step propagates all
-of them with one complex GEMM on the FP64 matrix cores.  Trajectory i draws from the reference's
+268-290, on the device (llsv_lplan_create_h: H_eff by one GEMM, expm by GEMMs); the trajectories
+then run together on the GPU (llsv_lplan_run): every step propagates all of them with one complex
+GEMM on the FP64 matrix cores.  Trajectory i draws from the reference's
 stream (std::mt19937_64 seeded with splitmix64(seed ^ splitmix64(i + 1))), so results follow the
 reference's draws; set `solver.rng = "philox"` for device-side draws instead.
 """
@@ -203,36 +204,31 @@ class TrajectorySolver:
         steps = max(1, int(math.ceil(duration / dt - 1e-9)))
         h = duration / steps
         dim = self._dim
-        eye = np.eye(dim, dtype=np.complex128)
         ops = []
         for ch in self._channels:
             if kind == Unraveling.PROJECTOR and ch["pauli"]:
-                j = math.sqrt(ch["g"]) * (eye - ch["p"])
+                j = math.sqrt(ch["g"]) * (np.eye(dim, dtype=np.complex128) - ch["p"])
             else:
                 j = math.sqrt(ch["gamma"]) * ch["l"]
             ops.append(np.ascontiguousarray(j))
         jump = kind != Unraveling.ANALOG
-        hnh = np.zeros((dim, dim), dtype=np.complex128)
-        if jump:
-            for j in ops:
-                hnh = hnh - 0.5j * _matmul(j.conj().T, j)
+        # H_nh = -(i/2) sum C^+ C, the propagator exp(-i (H + H_nh) h) (or the RK4 / Euler nodes
+        # -i (H(t_j) + H_nh)) are assembled on the device from H and the operators
         if self._h_fn is None:
-            nodes = [expm((-1j) * (self._h_const + hnh) * h)]
+            hs = self._h_const
         else:
             n_nodes = 2 * steps + 1 if jump else steps
             if n_nodes * dim * dim * 16.0 > 1.0e9:
                 _fail("TrajectorySolver", "time-dependent H(t) samples exceed 1 GB; increase dt")
-            nodes = []
-            for jn in range(n_nodes):
-                t = 0.5 * h * jn if jump else h * jn
-                nodes.append((-1j) * (self._eval_h(t) + hnh))
-        nodes = np.ascontiguousarray(np.stack(nodes))
+            hs = np.stack([self._eval_h(0.5 * h * jn if jump else h * jn) for jn in range(n_nodes)])
+        hs = np.ascontiguousarray(hs, dtype=np.complex128)
+        n_h = 1 if hs.ndim == 2 else hs.shape[0]
         opsa = np.ascontiguousarray(np.stack(ops)) if ops else np.zeros((1, dim, dim), dtype=np.complex128)
         lib = _native.lib()
         p = ctypes.c_void_p()
-        _native.check(lib.llsv_lplan_create(dim, 0 if jump else 1, steps, h, len(nodes),
-                                            nodes.ctypes.data_as(ctypes.c_void_p), len(ops),
-                                            opsa.ctypes.data_as(ctypes.c_void_p), ctypes.byref(p)))
+        _native.check(lib.llsv_lplan_create_h(dim, 0 if jump else 1, steps, h, n_h,
+                                              hs.ctypes.data_as(ctypes.c_void_p), len(ops),
+                                              opsa.ctypes.data_as(ctypes.c_void_p), ctypes.byref(p)))
         return _Plan(lib, p, steps, jump)
 
     def _run(self, plan, psi0, n, seed, rho_only=False):

@@ -175,3 +175,66 @@ def test_unitary_evolution_closed_form():
         assert np.max(np.abs(got - want)) < 1e-12
     with pytest.raises(ValueError):
         s.estimate_expectation(psi0, 1.0, 0.01, 4, np.kron(SM, np.eye(2)))
+
+
+def _run_plan(lib, plan, psi0, n, seed):
+    import ctypes
+
+    out = np.empty((n, psi0.size), dtype=np.complex128)
+    psi0 = np.ascontiguousarray(psi0, dtype=np.complex128)
+    rc = lib.llsv_lplan_run(plan, psi0.ctypes.data_as(ctypes.c_void_p), n, 0, seed, 0,
+                            out.ctypes.data_as(ctypes.c_void_p), None)
+    lib.llsv_lplan_destroy(plan)
+    assert rc == 0
+    return out
+
+
+@pytest.mark.parametrize("kind,timedep,n", [(0, False, 2), (1, False, 2), (2, False, 2), (0, True, 2),
+                                            (2, True, 2), (0, False, 7)])
+def test_device_plan_matches_host_plan(kind, timedep, n):
+    """llsv_lplan_create_h (H_eff, expm and nodes on the device) against llsv_lplan_create given
+    the oracle's host-built nodes: the same trajectories to rounding."""
+    import ctypes
+
+    from llsv import _native
+
+    lib = _native.lib()
+    rng = np.random.default_rng(40 + n)
+    d = 1 << n
+    if n == 2:
+        H, chans = two_qubit_model()
+    else:
+        a = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))
+        H = (a + a.conj().T) / (2 * math.sqrt(d))
+        chans = [(np.kron(np.eye(d // 2), SM), 0.5), (np.kron(SZ, np.eye(d // 2)), 0.3)]
+    Hf = (lambda t: H * math.cos(0.7 * t)) if timedep else H
+    t, dt = 0.4, 0.02
+    plan = LB.make_plan(Hf, chans, kind, t, dt)
+    jump = kind != LB.ANALOG
+    nodes = [plan["prop"]] if plan["constant"] else plan["nodes"]
+    nodes = np.ascontiguousarray(np.stack(nodes))
+    ops = np.ascontiguousarray(np.stack(plan["op"]))
+    if timedep:
+        hs = np.stack([Hf(0.5 * plan["h"] * j if jump else plan["h"] * j) for j in range(len(nodes))])
+    else:
+        hs = H
+    hs = np.ascontiguousarray(hs, dtype=np.complex128)
+    say(f"device plan kind={kind} timedep={timedep} n={n}")
+    ph, pd = ctypes.c_void_p(), ctypes.c_void_p()
+    k = 0 if jump else 1
+    _native.check(lib.llsv_lplan_create(d, k, plan["steps"], plan["h"], len(nodes),
+                                        nodes.ctypes.data_as(ctypes.c_void_p), len(ops),
+                                        ops.ctypes.data_as(ctypes.c_void_p), ctypes.byref(ph)))
+    _native.check(lib.llsv_lplan_create_h(d, k, plan["steps"], plan["h"], len(nodes),
+                                          hs.ctypes.data_as(ctypes.c_void_p), len(ops),
+                                          ops.ctypes.data_as(ctypes.c_void_p), ctypes.byref(pd)))
+    psi0 = rng.normal(size=d) + 1j * rng.normal(size=d)
+    psi0 /= np.linalg.norm(psi0)
+    a = _run_plan(lib, ph, psi0, 16, 77)
+    b = _run_plan(lib, pd, psi0, 16, 77)
+    assert np.max(np.abs(a - b)) < 1e-12
+    # argument checks
+    bad = ctypes.c_void_p()
+    assert lib.llsv_lplan_create_h(d, k, plan["steps"], plan["h"], 3, hs.ctypes.data_as(ctypes.c_void_p),
+                                   len(ops), ops.ctypes.data_as(ctypes.c_void_p), ctypes.byref(bad)) != 0
+    assert not bad.value
