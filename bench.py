@@ -381,11 +381,19 @@ def lindblad_leg(n=10, n_traj=4096, steps=20, dt=0.01):
     psi0 = rng.normal(size=d) + 1j * rng.normal(size=d)
     duration = steps * dt
     s.evolve_ensemble(psi0, duration, dt, 64, QT.Unraveling.STANDARD, 1)  # warm-up
+    # a whole call with the plan built afresh, then a repeated call (the solver keeps the plan of a
+    # constant H for the next call on the same time grid)
+    s._plan_cache = None
     t0 = time.perf_counter()
     rho = s.evolve_ensemble(psi0, duration, dt, n_traj, QT.Unraveling.STANDARD, 1)
     t_all = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rho2 = s.evolve_ensemble(psi0, duration, dt, n_traj, QT.Unraveling.STANDARD, 1)
+    t_rep = time.perf_counter() - t0
+    same = bool(np.array_equal(rho, rho2))
     # the two parts of the call: the plan (H_eff, exp(-i H_eff h) on the GPU, operator upload) and
     # the trajectories themselves
+    s._plan_cache = None
     t0 = time.perf_counter()
     plan = s._plan(QT.Unraveling.STANDARD, duration, dt)
     t_plan = time.perf_counter() - t0
@@ -396,11 +404,12 @@ def lindblad_leg(n=10, n_traj=4096, steps=20, dt=0.01):
     ms = QT.gemm_ms(n_traj, d, d, reps=20)
     flop = 8.0 * d * d * n_traj
     log(f"lindblad {n}q: {n_traj} trajectories x {steps} steps in {t_all:.3f} s (plan {t_plan:.3f} s, run "
-        f"{t_run:.3f} s); step GEMM {ms:.3f} ms "
+        f"{t_run:.3f} s; repeated call {t_rep:.3f} s, same rho {same}); step GEMM {ms:.3f} ms "
         f"= {flop / ms / 1e9:.1f} TFLOP/s")
     return {"workload": f"{n}-qubit dense Lindblad trajectories (dim {d}), {n_traj} trajectories x {steps} steps, "
                         "2 channels, STANDARD unraveling, reference random streams, complex128",
             "s_per_ensemble": round(t_all, 4), "plan_s": round(t_plan, 4), "run_s": round(t_run, 4),
+            "s_per_repeated_ensemble": round(t_rep, 4), "repeat_bitwise_equal": same,
             "trajectory_steps_per_s": round(n_traj * steps / t_run, 1),
             "gemm_ms_per_step": round(ms, 4), "gemm_TFLOPs": round(flop / ms / 1e9, 2),
             "gemm_frac_of_fp64_peak": round(flop / ms / 1e9 / 78.6, 3),

@@ -86,6 +86,7 @@ class TrajectorySolver:
         self._h_const = np.zeros((self._dim, self._dim), dtype=np.complex128)
         self._h_fn = None
         self._channels = []  # dicts: l, gamma, pauli, p, g
+        self._plan_cache = None  # ((kind, steps, h), _Plan) of the last constant-H plan
         self.rng = "reference"
         self.reset()
 
@@ -111,15 +112,17 @@ class TrajectorySolver:
             _fail(who, "non-finite H")
         if np.linalg.norm(h - h.conj().T) > 1e-9 * max(1.0, np.linalg.norm(h)):
             _fail(who, "H is not Hermitian")
-        return np.ascontiguousarray(h)
+        return np.array(h, dtype=np.complex128, order="C", copy=True)  # the model keeps its own copy
 
     def set_hamiltonian(self, H):
         if callable(H) and not isinstance(H, np.ndarray):
             self._check_h(H(0.0), "set_hamiltonian")
             self._h_fn = H
+            self._plan_cache = None
             return
         self._h_const = self._check_h(H, "set_hamiltonian")
         self._h_fn = None
+        self._plan_cache = None
 
     def add_lindblad(self, L, gamma: float):
         who = "add_lindblad"
@@ -133,14 +136,16 @@ class TrajectorySolver:
             _fail(who, "gamma must be finite and >= 0")
         if gamma == 0.0:
             return
-        ch = {"l": np.ascontiguousarray(l), "gamma": gamma, "pauli": False, "p": None, "g": 0.0}
+        ch = {"l": np.array(l, order="C", copy=True), "gamma": gamma, "pauli": False, "p": None, "g": 0.0}
         ok, p, c = self._pauli_like(l)
         if ok:
             ch.update(pauli=True, p=p, g=c * gamma)
         self._channels.append(ch)
+        self._plan_cache = None
 
     def clear_lindblad(self):
         self._channels = []
+        self._plan_cache = None
 
     def projector_applies(self, channel: int) -> bool:
         return bool(self._channels[int(channel)]["pauli"])
@@ -203,6 +208,11 @@ class TrajectorySolver:
             kind = Unraveling.STANDARD  # unitary evolution: exact propagator
         steps = max(1, int(math.ceil(duration / dt - 1e-9)))
         h = duration / steps
+        # a constant-H plan is kept for the next call with the same unraveling and time grid (the
+        # model setters drop it); a callable H is sampled afresh every call
+        key = (int(kind), steps, h)
+        if self._h_fn is None and self._plan_cache is not None and self._plan_cache[0] == key:
+            return self._plan_cache[1]
         dim = self._dim
         ops = []
         for ch in self._channels:
@@ -229,14 +239,17 @@ class TrajectorySolver:
         _native.check(lib.llsv_lplan_create_h(dim, 0 if jump else 1, steps, h, n_h,
                                               hs.ctypes.data_as(ctypes.c_void_p), len(ops),
                                               opsa.ctypes.data_as(ctypes.c_void_p), ctypes.byref(p)))
-        return _Plan(lib, p, steps, jump)
+        plan = _Plan(lib, p, steps, jump)
+        if self._h_fn is None:
+            self._plan_cache = (key, plan)
+        return plan
 
     def _run(self, plan, psi0, n, seed, rho_only=False):
         """Final states of trajectories 0 .. n-1 (n x dim), or with rho_only the sum of their
         |psi><psi| (dim x dim, formed on the device); batched so the host random table stays small."""
         d = self._dim
         out = None if rho_only else np.empty((n, d), dtype=np.complex128)
-        acc = np.zeros((d, d), dtype=np.complex128) if rho_only else None
+        acc = None
         part = np.empty((d, d), dtype=np.complex128) if rho_only else None
         per = (2 * plan.steps if plan.jump else plan.steps * max(1, self.n_channels)) * 8
         batch = max(1, min(n, (256 << 20) // max(per, 1)))
@@ -249,7 +262,10 @@ class TrajectorySolver:
             _native.check(plan.lib.llsv_lplan_run(plan.h, psi0.ctypes.data_as(ctypes.c_void_p), b, f,
                                                   int(seed) & ((1 << 64) - 1), rng, st, rs))
             if rho_only:
-                acc += part
+                if acc is None:
+                    acc, part = part, (np.empty_like(part) if f + b < n else None)
+                else:
+                    acc += part
         return acc if rho_only else out
 
     # -- evolution (src/TrajectorySolver.h:158-256)
@@ -271,7 +287,9 @@ class TrajectorySolver:
             _fail(who, "n_trajectories must be >= 1")
         if duration == 0.0:
             return np.outer(start, start.conj())
-        return self._run(self._plan(kind, duration, dt), start, n, seed, rho_only=True) / float(n)
+        rho = self._run(self._plan(kind, duration, dt), start, n, seed, rho_only=True)
+        rho /= float(n)
+        return rho
 
     def estimate_observable(self, psi0, duration, dt, n_trajectories, obs, kind=Unraveling.STANDARD, seed=1):
         who = "estimate_observable"

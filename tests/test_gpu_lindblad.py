@@ -238,3 +238,39 @@ def test_device_plan_matches_host_plan(kind, timedep, n):
     assert lib.llsv_lplan_create_h(d, k, plan["steps"], plan["h"], 3, hs.ctypes.data_as(ctypes.c_void_p),
                                    len(ops), ops.ctypes.data_as(ctypes.c_void_p), ctypes.byref(bad)) != 0
     assert not bad.value
+
+
+def test_plan_reuse_and_invalidation():
+    """A constant-H plan is reused by the next call on the same grid (same rho bit for bit) and
+    dropped by every model setter; the model keeps copies of H and the operators."""
+    import quantum_sim_trajectory as QT
+
+    H, chans = two_qubit_model()
+    Hu = H.copy()
+    s = solver(2, Hu, chans)
+    psi0 = np.array([0.6, 0.0, 0.8j, 0.0])
+    say("plan reuse")
+    a = s.evolve_ensemble(psi0, 0.5, 0.01, 32, QT.Unraveling.STANDARD, 3)
+    assert s._plan_cache is not None
+    b = s.evolve_ensemble(psi0, 0.5, 0.01, 32, QT.Unraveling.STANDARD, 3)
+    assert np.array_equal(a, b)
+    Hu *= 3.0  # the caller's array: not the model's copy
+    assert np.array_equal(a, s.evolve_ensemble(psi0, 0.5, 0.01, 32, QT.Unraveling.STANDARD, 3))
+    plan = LB.make_plan(H, chans, LB.PROJECTOR, 0.5, 0.01)
+    want = LB.ensemble_states(psi0, plan, 3, 32)
+    got = s.evolve_ensemble(psi0, 0.5, 0.01, 32, QT.Unraveling.PROJECTOR, 3)  # another unraveling
+    assert np.max(np.abs(got - (want.T @ want.conj()) / 32)) < 1e-9
+    s.set_hamiltonian(2.0 * H)
+    assert s._plan_cache is None
+    plan = LB.make_plan(2.0 * H, chans, LB.STANDARD, 0.5, 0.01)
+    want = LB.ensemble_states(psi0, plan, 3, 32)
+    got = s.evolve_ensemble(psi0, 0.5, 0.01, 32, QT.Unraveling.STANDARD, 3)
+    assert np.max(np.abs(got - (want.T @ want.conj()) / 32)) < 1e-9
+    s.add_lindblad(np.kron(SX, np.eye(2)), 0.2)
+    assert s._plan_cache is None
+    plan = LB.make_plan(2.0 * H, chans + [(np.kron(SX, np.eye(2)), 0.2)], LB.STANDARD, 0.5, 0.01)
+    want = LB.ensemble_states(psi0, plan, 3, 32)
+    got = s.evolve_ensemble(psi0, 0.5, 0.01, 32, QT.Unraveling.STANDARD, 3)
+    assert np.max(np.abs(got - (want.T @ want.conj()) / 32)) < 1e-9
+    s.clear_lindblad()
+    assert s._plan_cache is None
