@@ -236,17 +236,20 @@ __global__ void k_node_t(const v2d* __restrict__ H, int dim, const v2d* __restri
     }
 }
 
-// sums[j] = sum_i |X[i][j]|: the column sums of X^T's transpose, i.e. the row sums of the matrix
-// whose transpose X holds (the infinity norm's terms).
+// Column sums of |X| for a D x D matrix X (the row sums of the matrix whose transpose X holds: the
+// infinity norm's terms), in chunks of kColRows rows: sums[y][j] = sum over rows i of chunk y of
+// |X[i][j]|, for grid (ceil(D / 256), D / kColRows).  The caller adds the chunks.
+constexpr int kColRows = 64;
 __global__ void k_colabs(const v2d* __restrict__ X, int D, double* __restrict__ sums) {
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= D) return;
     double r = 0.0;
-    for (int i = 0; i < D; ++i) {
+    const int i0 = blockIdx.y * kColRows;
+    for (int i = i0; i < i0 + kColRows; ++i) {
         const v2d v = X[size_t(i) * D + j];
         r += hypot(v.x, v.y);
     }
-    sums[j] = r;
+    sums[size_t(blockIdx.y) * D + j] = r;
 }
 
 // Entries of a D x D block outside the leading dim x dim corner = 0.
@@ -581,12 +584,14 @@ int llsv_expm(int dim, const double* a, double* out) {
     // exp(A) by scaling and squaring: ||A / 2^s||_inf <= 1/2, 18-term Taylor series
     // (src/TrajectorySolver.h:97-111)
     double nrm = 0.0;
+    bool finite = true;
     for (int i = 0; i < dim; ++i) {
         double r = 0.0;
         for (int j = 0; j < dim; ++j) r += std::hypot(a[2 * (size_t(i) * dim + j)], a[2 * (size_t(i) * dim + j) + 1]);
+        finite = finite && std::isfinite(r);
         nrm = std::max(nrm, r);
     }
-    if (!std::isfinite(nrm)) return fail(LLSV_ERR_ARG, "llsv_expm: non-finite matrix");
+    if (!finite) return fail(LLSV_ERR_ARG, "llsv_expm: non-finite matrix");
     const int s = expm_power(nrm);
     const int D = ((std::max(dim, kBN) + kBN - 1) / kBN) * kBN;
     const uint64_t nn = uint64_t(D) * D;
@@ -813,17 +818,25 @@ int llsv_lplan_create_h(int dim, int kind, int64_t steps, double h, int n_h, con
         PK(res.alloc(blk));
         PK(term.alloc(blk));
         PK(tmp.alloc(blk));
-        PK(sums.alloc(size_t(D) * sizeof(double)));
+        const int chunks = D / kColRows;  // D is a multiple of 64
+        PK(sums.alloc(size_t(D) * chunks * sizeof(double)));
         PK(hipMemcpy(hraw.p, H, mat, hipMemcpyHostToDevice));
         hipLaunchKernelGGL(k_node_t, dim3(g), dim3(256), 0, st, hraw.as<const v2d>(), dim, GT, h, -0.5 * h,
                            a.as<v2d>(), D);
-        hipLaunchKernelGGL(k_colabs, dim3(unsigned((D + 255) / 256)), dim3(256), 0, st, a.as<const v2d>(), D,
-                           sums.as<double>());
+        hipLaunchKernelGGL(k_colabs, dim3(unsigned((D + 255) / 256), unsigned(chunks)), dim3(256), 0, st,
+                           a.as<const v2d>(), D, sums.as<double>());
         PK(hipGetLastError());
-        std::vector<double> hs(D);
-        PK(hipMemcpy(hs.data(), sums.p, size_t(D) * sizeof(double), hipMemcpyDeviceToHost));
-        const double nrm = *std::max_element(hs.begin(), hs.end());
-        if (!std::isfinite(nrm)) return fail(LLSV_ERR_ARG, "llsv_lplan_create_h: non-finite H or channels");
+        std::vector<double> hs(size_t(D) * chunks);
+        PK(hipMemcpy(hs.data(), sums.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
+        double nrm = 0.0;
+        bool finite = true;
+        for (int j = 0; j < D; ++j) {
+            double r = 0.0;
+            for (int y = 0; y < chunks; ++y) r += hs[size_t(y) * D + j];
+            finite = finite && std::isfinite(r);
+            nrm = std::max(nrm, r);
+        }
+        if (!finite) return fail(LLSV_ERR_ARG, "llsv_lplan_create_h: non-finite H or channels");
         PK(expm_dev(st, a, res, term, tmp, D, expm_power(nrm)));
         if (dim < D) hipLaunchKernelGGL(k_zero_pad, dim3(g), dim3(256), 0, st, res.as<v2d>(), dim, D);
         PK(hipGetLastError());

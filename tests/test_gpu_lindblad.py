@@ -274,3 +274,24 @@ def test_plan_reuse_and_invalidation():
     assert np.max(np.abs(got - (want.T @ want.conj()) / 32)) < 1e-9
     s.clear_lindblad()
     assert s._plan_cache is None
+
+
+def test_non_finite_inputs_rejected():
+    import ctypes
+
+    import quantum_sim_trajectory as QT
+    from llsv import _native
+
+    lib = _native.lib()
+    a = np.eye(4, dtype=complex)
+    a[1, 2] = np.nan
+    with pytest.raises(Exception):
+        QT.expm(a)
+    d = 64
+    H = np.eye(d, dtype=complex)
+    H[3, 5] = np.nan
+    ops = np.ascontiguousarray(np.eye(d, dtype=complex)[None])
+    p = ctypes.c_void_p()
+    rc = lib.llsv_lplan_create_h(d, 0, 4, 0.01, 1, H.ctypes.data_as(ctypes.c_void_p), 1,
+                                 ops.ctypes.data_as(ctypes.c_void_p), ctypes.byref(p))
+    assert rc != 0 and not p.value
