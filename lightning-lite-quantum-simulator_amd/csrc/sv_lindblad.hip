@@ -556,6 +556,20 @@ hipError_t expm_dev(hipStream_t st, DevBuf& a, DevBuf& res, DevBuf& term, DevBuf
     return e;
 }
 
+// Makes `dev` the current device for a scope (a plan's buffers live on the device it was made on).
+struct DeviceScope {
+    int prev = -1;
+    explicit DeviceScope(int dev) {
+        int cur = -1;
+        if (hipGetDevice(&cur) == hipSuccess && cur != dev && hipSetDevice(dev) == hipSuccess) prev = cur;
+    }
+    ~DeviceScope() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+    DeviceScope(const DeviceScope&) = delete;
+    DeviceScope& operator=(const DeviceScope&) = delete;
+};
+
 // Rows 0 .. T-1 of X (Tp rows of D) = src, rows T .. Tp-1 = 0.
 __global__ void k_fill_rows(v2d* __restrict__ X, const v2d* __restrict__ src, uint64_t T, uint64_t Tp, int D) {
     for (uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x; i < Tp * uint64_t(D); i += uint64_t(gridDim.x) * 256)
@@ -566,6 +580,7 @@ __global__ void k_fill_rows(v2d* __restrict__ X, const v2d* __restrict__ src, ui
 }  // namespace llsv
 
 struct llsv_lplan {
+    int device = 0;  // the device every buffer below lives on (current at creation)
     int dim = 0, D = 0, kind = 0, nch = 0, n_nodes = 0;
     int64_t steps = 0;
     double h = 0.0;
@@ -677,6 +692,7 @@ int llsv_lplan_create(int dim, int kind, int64_t steps, double h, int n_nodes, c
     if (!nodes || (n_nodes != 1 && n_nodes != want))
         return fail(LLSV_ERR_ARG, "llsv_lplan_create: %d nodes given, 1 or %lld expected", n_nodes, (long long)want);
     auto p = std::make_unique<llsv_lplan>();
+    if (hipGetDevice(&p->device) != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_lplan_create: no device");
     p->dim = dim;
     p->D = pad_dim(dim);
     p->kind = kind;
@@ -763,6 +779,7 @@ int llsv_lplan_create_h(int dim, int kind, int64_t steps, double h, int n_h, con
         return fail(LLSV_ERR_ARG, "llsv_lplan_create_h: %d Hamiltonians given, 1 or %lld expected", n_h,
                     (long long)want);
     auto p = std::make_unique<llsv_lplan>();
+    if (hipGetDevice(&p->device) != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_lplan_create_h: no device");
     p->dim = dim;
     p->D = pad_dim(dim);
     p->kind = kind;
@@ -857,12 +874,17 @@ int llsv_lplan_create_h(int dim, int kind, int64_t steps, double h, int n_h, con
     return LLSV_OK;
 }
 
-void llsv_lplan_destroy(llsv_lplan* p) { delete p; }
+void llsv_lplan_destroy(llsv_lplan* p) {
+    if (!p) return;
+    llsv::DeviceScope on(p->device);
+    delete p;
+}
 
 int llsv_lplan_run(llsv_lplan* p, const double* psi0, int64_t n_traj, int64_t first_traj, uint64_t seed, int rng,
                    double* states, double* rho_sum) {
     using namespace llsv;
     if (!p) return fail(LLSV_ERR_HANDLE, "llsv_lplan_run: null plan");
+    DeviceScope on(p->device);
     if (!psi0 || (!states && !rho_sum) || n_traj < 1 || first_traj < 0 || (rng != 0 && rng != 1))
         return fail(LLSV_ERR_ARG, "llsv_lplan_run: bad arguments");
     const int dim = p->dim, D = p->D, nch = p->nch;

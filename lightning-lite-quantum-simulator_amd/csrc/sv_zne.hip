@@ -420,7 +420,8 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
     std::lock_guard<std::mutex> lock(mu);
     static ZBuf b_apps, b_w, b_vals, b_amps;
     static ZBuf b_ev[2], b_off[2];
-    static int owner = -1;  // the device the cached buffers live on
+    static hipEvent_t done[2] = {nullptr, nullptr};
+    static int owner = -1;  // the device the cached buffers and events live on
     {
         int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_noisy_run: no device");
@@ -429,6 +430,10 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
                 if (b->p) (void)hipFree(b->p);
                 b->p = nullptr;
                 b->cap = 0;
+            }
+            for (hipEvent_t& e : done) {
+                if (e) (void)hipEventDestroy(e);
+                e = nullptr;
             }
             owner = dev;
         }
@@ -462,7 +467,6 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
     if (z->rng == 0) chunk = std::min<uint64_t>(chunk, 2048);
     if (amps) ZK(b_amps.need(size_t(std::min(chunk, N) * dim * 16)));
     static PinBuf h_ev[2], h_off[2];
-    static hipEvent_t done[2] = {nullptr, nullptr};
     for (int k = 0; k < 2; ++k)
         if (!done[k]) ZK(hipEventCreateWithFlags(&done[k], hipEventDisableTiming));
     std::vector<uint2> ev;
