@@ -409,7 +409,9 @@ void llsv_lplan_destroy(llsv_lplan* p);
  * std::uniform_real_distribution<double>(0, 1) (jumps) or std::normal_distribution<double>(0,
  * sqrt(h)) (diffusive), like TrajectorySolver::make_rng (src/TrajectorySolver.h:311-313), so an
  * ensemble reproduces the reference's draws; rng 1: Philox4x32-10 on the device.  A vanishing or
- * diverging norm is LLSV_ERR_NUMERIC with the reference's message. */
+ * diverging norm is LLSV_ERR_NUMERIC with the reference's message.  A plan keeps its working
+ * buffers between runs, so one plan must not run on two threads at once (two plans may); it runs
+ * on the device that was current when it was made, whatever device is current at the call. */
 int llsv_lplan_run(llsv_lplan* p, const double* psi0, int64_t n_traj, int64_t first_traj, uint64_t seed, int rng,
                    double* states, double* rho_sum);
 
