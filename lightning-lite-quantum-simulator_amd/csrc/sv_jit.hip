@@ -237,6 +237,13 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
 
     uint32_t S = 0xF00u;  // register bits of the current layout (the kernel starts with 8..11)
     bool scaled = false;
+    // RO_1QT diagonals (RZ) defer what they can into the lane's factor (sr, si), which every
+    // amplitude of the lane shares: on a lane bit the whole diagonal entry, on a register bit d0
+    // (the registers with the bit set take d1 / d0).  A factor that differs between lanes goes onto the
+    // amplitudes before they change lanes (lane_factor marks it); the pass's last step applies the
+    // rest.  Inner products of the gradients see |factor|^2 = 1 (unit-modulus entries, the same
+    // factor on psi and lambda), so they read the deferred state as it is.
+    bool lane_factor = false;
     int gi = 0;
     for (int i = 0; i < nops; ++i) {
         const RegOp& op = ops[i];
@@ -261,7 +268,22 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
                     o.line("{ const double* m = M + %zu;", c);
                 else
                     o.line("{ const double* m = M + %zu;", t8);
-                if (form == 3) {
+                if (form == 3 && op.kind == RO_1QT) {
+                    scaled = true;
+                    if (op.tmask) {
+                        o.line("const bool hi = (lt & %uu) != 0u;", op.tmask);
+                        o.line("r_phase(sr, si, hi ? m[6] : m[0], hi ? m[7] : m[1]);");
+                        lane_factor = true;
+                    } else {
+                        // the form-3 coefficients are RZ's, |d0| = 1: d1 / d0 = d1 conj(d0)
+                        o.line("const double rr = __builtin_fma(m[6], m[0], m[7] * m[1]), "
+                               "ri = __builtin_fma(m[7], m[0], -(m[6] * m[1]));");
+                        for (int k = 0; k < na; ++k)
+                            for (int j = 0; j < 16; ++j)
+                                if ((op.rmask >> j) & 1u) o.line("r_sc(%s[%d], rr, ri);", arrs[k], j);
+                        o.line("r_phase(sr, si, m[0], m[1]);");
+                    }
+                } else if (form == 3) {
                     if (op.tmask) {
                         o.line("const bool hi = (lt & %uu) != 0u;", op.tmask);
                         o.line("const double dr = hi ? m[6] : m[0], di = hi ? m[7] : m[1];");
@@ -384,6 +406,13 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
                        op.a, c + 6, c, c + 7, c + 1);
                 break;
             case RO_LAYOUT: {
+                if (lane_factor) {
+                    // the lanes' factors go onto their amplitudes before these change lanes
+                    o.line("#pragma unroll");
+                    o.line("for (int j = 0; j < 16; ++j) { r_sc(a[j], sr, si);%s }", sh.two ? " r_sc(l[j], sr, si);" : "");
+                    o.line("sr = 1.0; si = 0.0;");
+                    lane_factor = false;
+                }
                 // one state at a time through the 64 KiB exchange buffer
                 o.line("{ const unsigned int nl = r_ins0(r_ins0(r_ins0(r_ins0(tid, %d), %d), %d), %d);", op.q[0], op.q[1],
                        op.q[2], op.q[3]);
