@@ -677,13 +677,21 @@ def main():
     }
     del sv
     if not a.no_extra:
-        line["latency12"] = latency_leg(V3)
-        line["layered28"] = layered_leg(V3)
-        line["adjoint20"] = adjoint_leg()
-        line["trajectories14"] = trajectory_leg()
-        line["zne12"] = zne_leg()
-        line["lindblad10"] = lindblad_leg()
-        line["q32"] = q32_leg(V3)
+        def run_leg(name, fn):
+            # a leg that raises is recorded in the line; the sweep measurement above stands
+            try:
+                return fn()
+            except Exception as e:  # noqa: BLE001
+                log(f"{name} failed: {e!r}")
+                return {"error": repr(e)}
+
+        line["latency12"] = run_leg("latency12", lambda: latency_leg(V3))
+        line["layered28"] = run_leg("layered28", lambda: layered_leg(V3))
+        line["adjoint20"] = run_leg("adjoint20", adjoint_leg)
+        line["trajectories14"] = run_leg("trajectories14", trajectory_leg)
+        line["zne12"] = run_leg("zne12", zne_leg)
+        line["lindblad10"] = run_leg("lindblad10", lindblad_leg)
+        line["q32"] = run_leg("q32", lambda: q32_leg(V3))
     if not a.no_cpu_baseline:
         log("cpu baseline (subprocess)")
         line["cpu_baseline"] = cpu_baseline(n)
