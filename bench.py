@@ -262,6 +262,12 @@ def trajectory_leg(n=14, count=4096, mixed=2048):
         "s": round(t_graph, 4),
         "trajectory_gates_per_s": round(count * len(gates) / t_graph, 1),
         "trajectories_per_s": round(count / t_graph, 1),
+        # units of the plan for this config: a gate or noise site on one trajectory moves 32 * 2^n
+        # bytes if streamed (the states stay on chip, so this exceeds the HBM peak); a one-qubit
+        # gate is 14 * 2^n flops (the noise sites' arithmetic is left out)
+        "effective_GBps_if_streamed": round(count * (len(gates) + sites) * 32.0 * (1 << n) / t_graph / 1e9, 1),
+        "fp64_fraction_one_qubit_gates": round(
+            count * sum(1 for g in gates if g[0] != "cnot") * 14.0 * (1 << n) / t_graph / 78.6e12, 4),
         "jumps": int(j.sum()),
         "tuned_plan": {"threads": threads, "workgroups": workgroups, "fused": fused,
                        "tuning_s": round(tune_ms / 1e3, 3), "plan_create_s": round(t_plan, 3),
