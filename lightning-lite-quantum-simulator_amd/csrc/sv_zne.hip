@@ -7,10 +7,11 @@
 // from counter-based Philox draws made on the device.  Nothing is normalised and nothing waits on
 // a probability, so a trajectory is a plain gate sweep: each one lives in LDS (2^n x 16 B, n <= 12)
 // and owns L = min(512, 2^(n-1)) threads of a 512-thread workgroup (512 / L trajectories per
-// workgroup).  Per gate application: one pass over the gate's amplitude pairs (one-qubit gates),
-// quads (cnot, two-qubit mcz) or single amplitudes (wider mcz); the sampled Pauli acts on the same
-// qubits and is applied in the same pass (X^x Z^z on the pair or quad), a workgroup barrier ends
-// the pass.  A wider mcz takes a second pass for its per-qubit Paulis.  At the end each group
+// workgroup); from 13 qubits (up to 20) a trajectory's state is the workgroup's own slot in device
+// memory instead, one trajectory per workgroup.  Per gate application: one pass over the gate's
+// amplitude pairs (one-qubit gates), quads (cnot, two-qubit mcz) or single amplitudes (wider
+// mcz); the sampled Pauli acts on the same qubits and is applied in the same pass (X^x Z^z on the
+// pair or quad), a workgroup barrier ends the pass.  A wider mcz takes a second pass for its per-qubit Paulis.  At the end each group
 // reduces sum_i w_i |psi_i|^2 for its trajectory and may copy its state out.
 #include <hip/hip_runtime.h>
 
@@ -29,6 +30,7 @@ namespace llsv {
 namespace {
 
 constexpr int kZThreads = 512;
+constexpr int kZLdsQubits = 12;  // states up to 2^12 amplitudes (64 KiB) live in LDS
 
 // One gate application of the folded circuit.
 struct ZApp {
@@ -112,18 +114,21 @@ __device__ double group_sum(double v, int L, double* red) {
     return s;
 }
 
-template <bool PHILOX>
+// SLOT: the state is slot blockIdx.x of `slots` (2^n amplitudes each) instead of LDS; then lgL = 9
+// (one trajectory per workgroup).
+template <bool PHILOX, bool SLOT>
 __global__ __launch_bounds__(kZThreads) void k_zne(const ZApp* __restrict__ apps, int S, int n, int lgL,
                                                    uint64_t n_traj, const uint2* __restrict__ ev,
                                                    const uint64_t* __restrict__ ev_off, double p, uint64_t seed,
                                                    uint64_t traj_base, const double* __restrict__ w,
-                                                   double* __restrict__ vals, v2d* __restrict__ amps) {
+                                                   double* __restrict__ vals, v2d* __restrict__ amps,
+                                                   v2d* __restrict__ slots) {
     extern __shared__ v2d lds_states[];
     __shared__ double red[kZThreads / 64];
     const int L = 1 << lgL, G = kZThreads >> lgL;
     const int g = int(threadIdx.x) >> lgL, r = int(threadIdx.x) & (L - 1);
     const uint32_t dim = 1u << n;
-    v2d* s = lds_states + size_t(g) * dim;
+    v2d* s = SLOT ? slots + size_t(blockIdx.x) * dim : lds_states + size_t(g) * dim;
     for (uint64_t tb = uint64_t(blockIdx.x) * G; tb < n_traj; tb += uint64_t(gridDim.x) * G) {
         const uint64_t t = tb + uint64_t(g);
         const bool live = t < n_traj;
@@ -412,13 +417,15 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
     const int S = int(apps.size());
     const uint64_t N = uint64_t(n_traj), dim = uint64_t(1) << n;
 
-    // launch shape: L threads per trajectory, kZThreads / L trajectories per workgroup
+    // launch shape: L threads per trajectory, kZThreads / L trajectories per workgroup; above 12
+    // qubits one trajectory per workgroup, in a device-memory slot
+    const bool in_slot = n > kZLdsQubits;
     int lgL = std::min(9, std::max(0, n - 1));
     const int G = kZThreads >> lgL;
-    const size_t lds = size_t(G) * dim * 16;
+    const size_t lds = in_slot ? 0 : size_t(G) * dim * 16;
     static std::mutex mu;  // the device buffers below are shared by every handle
     std::lock_guard<std::mutex> lock(mu);
-    static ZBuf b_apps, b_w, b_vals, b_amps;
+    static ZBuf b_apps, b_w, b_vals, b_amps, b_slots;
     static ZBuf b_ev[2], b_off[2];
     static hipEvent_t done[2] = {nullptr, nullptr};
     static int owner = -1;  // the device the cached buffers and events live on
@@ -426,7 +433,7 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
         int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_noisy_run: no device");
         if (owner != dev) {  // another device is current: the buffers are allocated again there
-            for (ZBuf* b : {&b_apps, &b_w, &b_vals, &b_amps, &b_ev[0], &b_ev[1], &b_off[0], &b_off[1]}) {
+            for (ZBuf* b : {&b_apps, &b_w, &b_vals, &b_amps, &b_slots, &b_ev[0], &b_ev[1], &b_off[0], &b_off[1]}) {
                 if (b->p) (void)hipFree(b->p);
                 b->p = nullptr;
                 b->cap = 0;
@@ -441,9 +448,9 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
     static bool attr_set = false;
     if (!attr_set) {
         // up to 64 KiB of dynamic LDS (one 12-qubit state)
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zne<false>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zne<false, false>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zne<true>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zne<true, false>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
         attr_set = true;
     }
@@ -466,6 +473,15 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
     if (amps) chunk = std::max<uint64_t>(1, (uint64_t(256) << 20) / (dim * 16));
     if (z->rng == 0) chunk = std::min<uint64_t>(chunk, 2048);
     if (amps) ZK(b_amps.need(size_t(std::min(chunk, N) * dim * 16)));
+    // device-memory states (above 12 qubits): one slot per workgroup, at most 2 per CU and 2 GiB
+    unsigned slot_blocks = 0;
+    if (in_slot) {
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        const uint64_t cap = std::max<uint64_t>(1, (uint64_t(2) << 30) / (dim * 16));
+        slot_blocks = unsigned(std::min<uint64_t>({uint64_t(2) * unsigned(cus), cap, std::min(chunk, N)}));
+        ZK(b_slots.need(size_t(slot_blocks) * dim * 16));
+    }
     static PinBuf h_ev[2], h_off[2];
     for (int k = 0; k < 2; ++k)
         if (!done[k]) ZK(hipEventCreateWithFlags(&done[k], hipEventDisableTiming));
@@ -474,9 +490,9 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
     int c = 0;
     for (uint64_t t0 = 0; t0 < N; t0 += chunk, ++c) {
         const uint64_t nt = std::min(chunk, N - t0);
-        const int slot = c & 1;
+        const int ev_slot = c & 1;
         if (z->rng == 0) {
-            if (c >= 2) ZK(hipEventSynchronize(done[slot]));  // chunk c - 2 no longer reads this slot
+            if (c >= 2) ZK(hipEventSynchronize(done[ev_slot]));  // chunk c - 2 no longer reads these buffers
             // the reference stream, trajectory by trajectory, site by site
             ev.clear();
             off.assign(nt + 1, 0);
@@ -489,30 +505,41 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
                 off[t + 1] = ev.size();
             }
             const size_t evb = sizeof(uint2) * std::max<size_t>(1, ev.size()), offb = sizeof(uint64_t) * off.size();
-            ZK(h_ev[slot].need(evb));
-            ZK(h_off[slot].need(offb));
-            if (!ev.empty()) std::memcpy(h_ev[slot].p, ev.data(), sizeof(uint2) * ev.size());
-            std::memcpy(h_off[slot].p, off.data(), offb);
-            ZK(b_ev[slot].need(evb));
-            ZK(b_off[slot].need(offb));
-            if (!ev.empty()) ZK(hipMemcpyAsync(b_ev[slot].p, h_ev[slot].p, sizeof(uint2) * ev.size(), hipMemcpyHostToDevice, nullptr));
-            ZK(hipMemcpyAsync(b_off[slot].p, h_off[slot].p, offb, hipMemcpyHostToDevice, nullptr));
+            ZK(h_ev[ev_slot].need(evb));
+            ZK(h_off[ev_slot].need(offb));
+            if (!ev.empty()) std::memcpy(h_ev[ev_slot].p, ev.data(), sizeof(uint2) * ev.size());
+            std::memcpy(h_off[ev_slot].p, off.data(), offb);
+            ZK(b_ev[ev_slot].need(evb));
+            ZK(b_off[ev_slot].need(offb));
+            if (!ev.empty()) ZK(hipMemcpyAsync(b_ev[ev_slot].p, h_ev[ev_slot].p, sizeof(uint2) * ev.size(), hipMemcpyHostToDevice, nullptr));
+            ZK(hipMemcpyAsync(b_off[ev_slot].p, h_off[ev_slot].p, offb, hipMemcpyHostToDevice, nullptr));
         }
-        const unsigned grid = unsigned(std::min<uint64_t>((nt + G - 1) / G, 16384));
+        const unsigned grid = unsigned(std::min<uint64_t>((nt + G - 1) / G, in_slot ? slot_blocks : 16384));
         // w is null unless weights were given (the cached buffer may hold an earlier call's)
         const double* dw = weights ? static_cast<const double*>(b_w.p) : nullptr;
         double* dvals = weights ? static_cast<double*>(b_vals.p) + t0 : nullptr;
         v2d* damps = amps ? static_cast<v2d*>(b_amps.p) : nullptr;
-        if (z->rng == 0)
-            hipLaunchKernelGGL(k_zne<false>, dim3(grid), dim3(kZThreads), lds, nullptr, static_cast<const ZApp*>(b_apps.p), S,
-                               n, lgL, nt, static_cast<const uint2*>(b_ev[slot].p),
-                               static_cast<const uint64_t*>(b_off[slot].p), z->p, z->seed, 0, dw, dvals, damps);
-        else
-            hipLaunchKernelGGL(k_zne<true>, dim3(grid), dim3(kZThreads), lds, nullptr, static_cast<const ZApp*>(b_apps.p), S,
-                               n, lgL, nt, nullptr, nullptr, z->p, z->seed, z->philox_next + t0,
-                               dw, dvals, damps);
+        v2d* dslots = static_cast<v2d*>(b_slots.p);
+        const ZApp* dapps = static_cast<const ZApp*>(b_apps.p);
+        if (z->rng == 0) {
+            const uint2* dev_ev = static_cast<const uint2*>(b_ev[ev_slot].p);
+            const uint64_t* dev_off = static_cast<const uint64_t*>(b_off[ev_slot].p);
+            if (in_slot)
+                hipLaunchKernelGGL((k_zne<false, true>), dim3(grid), dim3(kZThreads), 0, nullptr, dapps, S, n, lgL, nt,
+                                   dev_ev, dev_off, z->p, z->seed, uint64_t(0), dw, dvals, damps, dslots);
+            else
+                hipLaunchKernelGGL((k_zne<false, false>), dim3(grid), dim3(kZThreads), lds, nullptr, dapps, S, n, lgL, nt,
+                                   dev_ev, dev_off, z->p, z->seed, uint64_t(0), dw, dvals, damps, dslots);
+        } else {
+            if (in_slot)
+                hipLaunchKernelGGL((k_zne<true, true>), dim3(grid), dim3(kZThreads), 0, nullptr, dapps, S, n, lgL, nt,
+                                   nullptr, nullptr, z->p, z->seed, z->philox_next + t0, dw, dvals, damps, dslots);
+            else
+                hipLaunchKernelGGL((k_zne<true, false>), dim3(grid), dim3(kZThreads), lds, nullptr, dapps, S, n, lgL, nt,
+                                   nullptr, nullptr, z->p, z->seed, z->philox_next + t0, dw, dvals, damps, dslots);
+        }
         ZK(hipGetLastError());
-        if (z->rng == 0) ZK(hipEventRecord(done[slot], nullptr));
+        if (z->rng == 0) ZK(hipEventRecord(done[ev_slot], nullptr));
         if (amps) ZK(hipMemcpy(amps + 2 * t0 * dim, damps, size_t(nt * dim * 16), hipMemcpyDeviceToHost));
     }
     if (weights) ZK(hipMemcpy(values, b_vals.p, sizeof(double) * N, hipMemcpyDeviceToHost));

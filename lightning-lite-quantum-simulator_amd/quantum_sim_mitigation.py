@@ -8,9 +8,10 @@ src/Mitigation.cpp).
   order), kept in libllsv (llsv_noisy_*), so one seed gives the reference's Pauli insertions
   whether gates are applied one at a time or trajectories run in a batch.
 * `ZNE(trajectories).extrapolate(noisy, circuit_fn, observable_fn, scales)` (src/Mitigation.h:
-  340-390): for states of up to 12 qubits the circuit is recorded once and every scale's
-  trajectories run in one GPU launch (llsv_noisy_run), each trajectory's state in LDS; a
-  `Diagonal` observable is evaluated in the kernel, any other callable on each final state.
+  340-390): for states of up to 20 qubits the circuit is recorded once and every scale's
+  trajectories run in one GPU launch (llsv_noisy_run), each trajectory's state in LDS (up to 12
+  qubits) or in a per-workgroup device-memory slot; a `Diagonal` observable is evaluated in the
+  kernel, any other callable on each final state.
   Larger states run trajectory by trajectory through `noisy` exactly like the reference's loop.
   Mean, standard error (Welford, in trajectory order), Richardson weights and the weighted
   error follow src/Mitigation.h:357-390.
@@ -36,7 +37,7 @@ MAX_ZNE_SCALES = 8
 MAX_ZNE_SCALE = 99
 DEFAULT_ZNE_TRAJECTORIES = 1000
 MAX_DM_QUBITS = 10
-MAX_BATCH_QUBITS = 12  # LLSV_ZNE_MAX_QUBITS
+MAX_BATCH_QUBITS = 20  # LLSV_ZNE_MAX_QUBITS
 
 
 class _NQ(int):
@@ -328,7 +329,7 @@ class _FinalState:
 
 def run_trajectories(noisy: NoisyBackend, n: int, gates, scale: int, n_traj: int, observable):
     """Per-trajectory observable values of n_traj trajectories at one scale, batched on the GPU
-    (n <= 12), continuing `noisy`'s stream."""
+    (n <= 20), continuing `noisy`'s stream."""
     arr = pack_gates(gates)
     L = noisy._z.lib
     vals = np.empty(int(n_traj), dtype=np.float64)

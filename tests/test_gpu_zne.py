@@ -54,7 +54,8 @@ def gpu_values(n, gates, p, seed, scale, n_traj, w, rng="reference", noisy=None)
 
 
 @pytest.mark.parametrize("n,G,p,seed", [(1, 6, 0.3, 1), (2, 10, 0.2, 2), (3, 15, 0.2, 3), (5, 30, 0.1, 4),
-                                        (8, 40, 0.05, 5), (10, 30, 0.05, 6), (12, 24, 0.04, 7)])
+                                        (8, 40, 0.05, 5), (10, 30, 0.05, 6), (12, 24, 0.04, 7),
+                                        (13, 20, 0.04, 8), (15, 14, 0.03, 9)])
 def test_reference_stream_matches_oracle(n, G, p, seed):
     # every trajectory: the oracle's Pauli insertions (same mt19937_64 stream), the same state
     gates = random_circuit(n, G, seed)
@@ -202,8 +203,16 @@ def test_errors_and_size_limits():
     QM.ZNE(5).extrapolate(nb, lambda b: b.h(0), QM.Diagonal(w_zz()))
     assert nb.scale == 3
     with pytest.raises(ValueError):
-        gpu_values(13, [("h", 0)], 0.1, 1, 1, 2, np.ones(2 ** 13))
-    # 13 qubits: extrapolate runs trajectory by trajectory through the backend
+        gpu_values(21, [("h", 0)], 0.1, 1, 1, 2, np.ones(2 ** 21))
+    # 21 qubits: extrapolate runs trajectory by trajectory through the backend
+    n = 21
+    nb21 = QM.NoisyBackend(QM.ReferenceBackend(n), 0.2, 4)
+    g21 = [("h", 0), ("cnot", 0, 20)]
+    r = QM.ZNE(2).extrapolate(nb21, lambda b: [_apply(b, g) for g in g21],
+                              lambda b: float(b.probabilities()[0]), [1, 3])
+    o = Z.extrapolate(n, g21, 0.2, 4, 2, [1, 3], np.eye(1, 2 ** n, 0).ravel())
+    assert abs(r.value - o["value"]) < 1e-12
+    # 13 qubits: the batched path with device-memory states
     n = 13
     nb13 = QM.NoisyBackend(QM.ReferenceBackend(n), 0.1, 4)
     gates = [("h", 0), ("cnot", 0, 12), ("ry", 5, 0.3)]
