@@ -322,11 +322,12 @@ def latency_leg(V3, n=12, n_ops=1000, seeds=10):
 
 
 def zne_leg(n=12, layers=4, p=0.005, trajectories=8192):
-    """Zero-noise extrapolation of <sum_i Z_i> / n on a 12-qubit circuit (4 layers of RY + RZ on
-    every qubit and a CNOT ladder, 140 gates), depolarizing p = 0.005 per site, scales 1, 3, 5,
-    8192 trajectories per scale, every trajectory's state in LDS: with the reference's noise
-    stream (mt19937_64 draws on the host, the same Pauli insertions as the reference's
-    NoisyBackend for the same seed) and with device Philox draws."""
+    """Zero-noise extrapolation of <sum_i Z_i> / n on an n-qubit circuit (layers of RY + RZ on every
+    qubit and a CNOT ladder), depolarizing p = 0.005 per site, scales 1, 3, 5: with the reference's
+    noise stream (mt19937_64 draws on the host, the same Pauli insertions as the reference's
+    NoisyBackend for the same seed) and with device Philox draws.  Leg zne12: 12 qubits, 4 layers
+    (140 gates), 8192 trajectories per scale, every state in LDS; leg zne16: 16 qubits, 2 layers
+    (94 gates), 1024 trajectories per scale, every state in a device-memory slot."""
     import quantum_sim_mitigation as QM
 
     rng = np.random.Generator(np.random.PCG64(3))
@@ -364,6 +365,10 @@ def zne_leg(n=12, layers=4, p=0.005, trajectories=8192):
     ideal = QM.ZNE(1).extrapolate(QM.NoisyBackend(QM.ReferenceBackend(n), 0.0, 1), circ, w, [1, 3]).value
     out["ideal"] = ideal
     return out
+
+
+def zne16_leg():
+    return zne_leg(n=16, layers=2, trajectories=1024)
 
 
 def lindblad_leg(n=10, n_traj=4096, steps=20, dt=0.01):
@@ -505,7 +510,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the latency, layered, adjoint, trajectory, ZNE, Lindblad and 32-qubit legs")
     ap.add_argument("--leg", choices=["latency12", "layered28", "program28", "adjoint20", "trajectories14", "zne12",
-                                      "lindblad10", "q32"],
+                                      "zne16", "lindblad10", "q32"],
                     help="run only this leg (for profiling) and print its JSON")
     ap.add_argument("--probe", action="store_true",
                     help="launch each kernel a few times and exit (for rocprofv3 counter runs)")
@@ -526,7 +531,7 @@ def main():
         fn = {"latency12": lambda: latency_leg(V3), "layered28": lambda: layered_leg(V3),
               "program28": lambda: program_leg(V3),
               "adjoint20": adjoint_leg, "trajectories14": trajectory_leg, "zne12": zne_leg,
-              "lindblad10": lindblad_leg,
+              "zne16": zne16_leg, "lindblad10": lindblad_leg,
               "q32": lambda: q32_leg(V3)}[a.leg]
         print(json.dumps({a.leg: fn()}), flush=True)
         return
@@ -696,6 +701,7 @@ def main():
         line["adjoint20"] = run_leg("adjoint20", adjoint_leg)
         line["trajectories14"] = run_leg("trajectories14", trajectory_leg)
         line["zne12"] = run_leg("zne12", zne_leg)
+        line["zne16"] = run_leg("zne16", zne16_leg)
         line["lindblad10"] = run_leg("lindblad10", lindblad_leg)
         line["q32"] = run_leg("q32", lambda: q32_leg(V3))
     if not a.no_cpu_baseline:

@@ -8,7 +8,7 @@
 // a probability, so a trajectory is a plain gate sweep: each one lives in LDS (2^n x 16 B, n <= 12)
 // and owns L = min(512, 2^(n-1)) threads of a 512-thread workgroup (512 / L trajectories per
 // workgroup); from 13 qubits (up to 20) a trajectory's state is the workgroup's own slot in device
-// memory instead, one trajectory per workgroup.  Per gate application: one pass over the gate's
+// memory instead, one trajectory per 1024-thread workgroup.  Per gate application: one pass over the gate's
 // amplitude pairs (one-qubit gates), quads (cnot, two-qubit mcz) or single amplitudes (wider
 // mcz); the sampled Pauli acts on the same qubits and is applied in the same pass (X^x Z^z on the
 // pair or quad), a workgroup barrier ends the pass.  A wider mcz takes a second pass for its per-qubit Paulis.  At the end each group
@@ -30,6 +30,7 @@ namespace llsv {
 namespace {
 
 constexpr int kZThreads = 512;
+constexpr int kZSlotThreads = 1024;  // one device-memory trajectory per workgroup
 constexpr int kZLdsQubits = 12;  // states up to 2^12 amplitudes (64 KiB) live in LDS
 
 // One gate application of the folded circuit.
@@ -100,7 +101,7 @@ __device__ __forceinline__ void code_masks(const ZApp& a, uint32_t code, uint32_
 
 __device__ __forceinline__ v2d neg_if(v2d v, bool neg) { return neg ? -v : v; }
 
-// Group-wide sum of L consecutive lanes (L a power of two) of a kZThreads-thread workgroup.
+// Group-wide sum of L consecutive lanes (L a power of two, L / 64 <= entries of red).
 __device__ double group_sum(double v, int L, double* red) {
     const int w = int(threadIdx.x) >> 6;
     for (int o = 1; o < L && o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
@@ -114,18 +115,19 @@ __device__ double group_sum(double v, int L, double* red) {
     return s;
 }
 
-// SLOT: the state is slot blockIdx.x of `slots` (2^n amplitudes each) instead of LDS; then lgL = 9
-// (one trajectory per workgroup).
+// SLOT: the state is slot blockIdx.x of `slots` (2^n amplitudes each) instead of LDS, worked on by a
+// kZSlotThreads-thread workgroup; then lgL = 10 (one trajectory per workgroup).
 template <bool PHILOX, bool SLOT>
-__global__ __launch_bounds__(kZThreads) void k_zne(const ZApp* __restrict__ apps, int S, int n, int lgL,
+__global__ __launch_bounds__(SLOT ? kZSlotThreads : kZThreads) void k_zne(const ZApp* __restrict__ apps, int S, int n, int lgL,
                                                    uint64_t n_traj, const uint2* __restrict__ ev,
                                                    const uint64_t* __restrict__ ev_off, double p, uint64_t seed,
                                                    uint64_t traj_base, const double* __restrict__ w,
                                                    double* __restrict__ vals, v2d* __restrict__ amps,
                                                    v2d* __restrict__ slots) {
     extern __shared__ v2d lds_states[];
-    __shared__ double red[kZThreads / 64];
-    const int L = 1 << lgL, G = kZThreads >> lgL;
+    constexpr int NT = SLOT ? kZSlotThreads : kZThreads;
+    __shared__ double red[NT / 64];
+    const int L = 1 << lgL, G = NT >> lgL;
     const int g = int(threadIdx.x) >> lgL, r = int(threadIdx.x) & (L - 1);
     const uint32_t dim = 1u << n;
     v2d* s = SLOT ? slots + size_t(blockIdx.x) * dim : lds_states + size_t(g) * dim;
@@ -420,8 +422,8 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
     // launch shape: L threads per trajectory, kZThreads / L trajectories per workgroup; above 12
     // qubits one trajectory per workgroup, in a device-memory slot
     const bool in_slot = n > kZLdsQubits;
-    int lgL = std::min(9, std::max(0, n - 1));
-    const int G = kZThreads >> lgL;
+    const int lgL = in_slot ? 10 : std::min(9, std::max(0, n - 1));
+    const int G = in_slot ? 1 : kZThreads >> lgL;
     const size_t lds = in_slot ? 0 : size_t(G) * dim * 16;
     static std::mutex mu;  // the device buffers below are shared by every handle
     std::lock_guard<std::mutex> lock(mu);
@@ -473,14 +475,19 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
     if (amps) chunk = std::max<uint64_t>(1, (uint64_t(256) << 20) / (dim * 16));
     if (z->rng == 0) chunk = std::min<uint64_t>(chunk, 2048);
     if (amps) ZK(b_amps.need(size_t(std::min(chunk, N) * dim * 16)));
-    // device-memory states (above 12 qubits): one slot per workgroup, at most 2 per CU and 2 GiB
+    // device-memory states (above 12 qubits): one slot per workgroup.  While half as many slots as
+    // CUs fit in 128 MiB (up to 16 qubits) that many run, so the slots stay in L2 / Infinity Cache
+    // (MI355X, 16 qubits: 7.7 TB/s of state traffic against 6.7 at two slots per CU); larger
+    // states stream from HBM and run one slot per CU (at most 4 GiB of slots).
     unsigned slot_blocks = 0;
     if (in_slot) {
         int dev = 0, cus = 256;
         if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        const uint64_t cap = std::max<uint64_t>(1, (uint64_t(2) << 30) / (dim * 16));
-        slot_blocks = unsigned(std::min<uint64_t>({uint64_t(2) * unsigned(cus), cap, std::min(chunk, N)}));
-        ZK(b_slots.need(size_t(slot_blocks) * dim * 16));
+        const uint64_t half = std::max(1u, unsigned(cus) / 2), slot_bytes = dim * 16;
+        const uint64_t want = half * slot_bytes <= (uint64_t(128) << 20) ? half : uint64_t(unsigned(cus));
+        const uint64_t cap = std::max<uint64_t>(1, (uint64_t(4) << 30) / slot_bytes);
+        slot_blocks = unsigned(std::min<uint64_t>({want, cap, std::min(chunk, N)}));
+        ZK(b_slots.need(size_t(slot_blocks) * slot_bytes));
     }
     static PinBuf h_ev[2], h_off[2];
     for (int k = 0; k < 2; ++k)
@@ -525,14 +532,14 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
             const uint2* dev_ev = static_cast<const uint2*>(b_ev[ev_slot].p);
             const uint64_t* dev_off = static_cast<const uint64_t*>(b_off[ev_slot].p);
             if (in_slot)
-                hipLaunchKernelGGL((k_zne<false, true>), dim3(grid), dim3(kZThreads), 0, nullptr, dapps, S, n, lgL, nt,
+                hipLaunchKernelGGL((k_zne<false, true>), dim3(grid), dim3(kZSlotThreads), 0, nullptr, dapps, S, n, lgL, nt,
                                    dev_ev, dev_off, z->p, z->seed, uint64_t(0), dw, dvals, damps, dslots);
             else
                 hipLaunchKernelGGL((k_zne<false, false>), dim3(grid), dim3(kZThreads), lds, nullptr, dapps, S, n, lgL, nt,
                                    dev_ev, dev_off, z->p, z->seed, uint64_t(0), dw, dvals, damps, dslots);
         } else {
             if (in_slot)
-                hipLaunchKernelGGL((k_zne<true, true>), dim3(grid), dim3(kZThreads), 0, nullptr, dapps, S, n, lgL, nt,
+                hipLaunchKernelGGL((k_zne<true, true>), dim3(grid), dim3(kZSlotThreads), 0, nullptr, dapps, S, n, lgL, nt,
                                    nullptr, nullptr, z->p, z->seed, z->philox_next + t0, dw, dvals, damps, dslots);
             else
                 hipLaunchKernelGGL((k_zne<true, false>), dim3(grid), dim3(kZThreads), lds, nullptr, dapps, S, n, lgL, nt,
