@@ -7,8 +7,9 @@
 // from counter-based Philox draws made on the device.  Nothing is normalised and nothing waits on
 // a probability, so a trajectory is a plain gate sweep: each one lives in LDS (2^n x 16 B, n <= 12)
 // and owns L = min(512, 2^(n-1)) threads of a 512-thread workgroup (512 / L trajectories per
-// workgroup); from 13 qubits (up to 20) a trajectory's state is the workgroup's own slot in device
-// memory instead, one trajectory per 1024-thread workgroup.  Per gate application: one pass over the gate's
+// workgroup); above 12 qubits (up to 20) one trajectory per 1024-thread workgroup, its state in
+// 128 KiB of LDS at 13 qubits (where the device allows that much per workgroup), else the
+// workgroup's own slot in device memory.  Per gate application: one pass over the gate's
 // amplitude pairs (one-qubit gates), quads (cnot, two-qubit mcz) or single amplitudes (wider
 // mcz); the sampled Pauli acts on the same qubits and is applied in the same pass (X^x Z^z on the
 // pair or quad), a workgroup barrier ends the pass.  A wider mcz takes a second pass for its per-qubit Paulis.  At the end each group
@@ -30,8 +31,8 @@ namespace llsv {
 namespace {
 
 constexpr int kZThreads = 512;
-constexpr int kZSlotThreads = 1024;  // one device-memory trajectory per workgroup
-constexpr int kZLdsQubits = 12;  // states up to 2^12 amplitudes (64 KiB) live in LDS
+constexpr int kZSlotThreads = 1024;  // above 12 qubits: one trajectory per workgroup
+constexpr int kZLdsQubits = 12;  // up to 2^12 amplitudes (64 KiB) per state: 512-thread workgroups
 
 // One gate application of the folded circuit.
 struct ZApp {
@@ -115,17 +116,16 @@ __device__ double group_sum(double v, int L, double* red) {
     return s;
 }
 
-// SLOT: the state is slot blockIdx.x of `slots` (2^n amplitudes each) instead of LDS, worked on by a
-// kZSlotThreads-thread workgroup; then lgL = 10 (one trajectory per workgroup).
-template <bool PHILOX, bool SLOT>
-__global__ __launch_bounds__(SLOT ? kZSlotThreads : kZThreads) void k_zne(const ZApp* __restrict__ apps, int S, int n, int lgL,
+// NT threads per workgroup, 2^lgL of them per trajectory.  SLOT: the state is slot blockIdx.x of
+// `slots` (2^n amplitudes each) instead of LDS (then NT = 1024, lgL = 10: one trajectory per workgroup).
+template <bool PHILOX, bool SLOT, int NT>
+__global__ __launch_bounds__(NT) void k_zne(const ZApp* __restrict__ apps, int S, int n, int lgL,
                                                    uint64_t n_traj, const uint2* __restrict__ ev,
                                                    const uint64_t* __restrict__ ev_off, double p, uint64_t seed,
                                                    uint64_t traj_base, const double* __restrict__ w,
                                                    double* __restrict__ vals, v2d* __restrict__ amps,
                                                    v2d* __restrict__ slots) {
     extern __shared__ v2d lds_states[];
-    constexpr int NT = SLOT ? kZSlotThreads : kZThreads;
     __shared__ double red[NT / 64];
     const int L = 1 << lgL, G = NT >> lgL;
     const int g = int(threadIdx.x) >> lgL, r = int(threadIdx.x) & (L - 1);
@@ -232,6 +232,28 @@ __global__ __launch_bounds__(SLOT ? kZSlotThreads : kZThreads) void k_zne(const 
             for (uint32_t i = uint32_t(r); i < dim; i += uint32_t(L)) amps[t * dim + i] = s[i];
         __syncthreads();
     }
+}
+
+// Arguments of one k_zne launch.
+struct ZArgs {
+    const ZApp* apps;
+    int S, n, lgL;
+    uint64_t n_traj;
+    const uint2* ev;
+    const uint64_t* ev_off;
+    double p;
+    uint64_t seed, traj_base;
+    const double* w;
+    double* vals;
+    v2d* amps;
+    v2d* slots;
+};
+
+template <bool PHILOX, bool SLOT, int NT>
+hipError_t launch_zne(unsigned grid, size_t lds, const ZArgs& a) {
+    hipLaunchKernelGGL((k_zne<PHILOX, SLOT, NT>), dim3(grid), dim3(NT), lds, nullptr, a.apps, a.S, a.n, a.lgL,
+                       a.n_traj, a.ev, a.ev_off, a.p, a.seed, a.traj_base, a.w, a.vals, a.amps, a.slots);
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -419,18 +441,13 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
     const int S = int(apps.size());
     const uint64_t N = uint64_t(n_traj), dim = uint64_t(1) << n;
 
-    // launch shape: L threads per trajectory, kZThreads / L trajectories per workgroup; above 12
-    // qubits one trajectory per workgroup, in a device-memory slot
-    const bool in_slot = n > kZLdsQubits;
-    const int lgL = in_slot ? 10 : std::min(9, std::max(0, n - 1));
-    const int G = in_slot ? 1 : kZThreads >> lgL;
-    const size_t lds = in_slot ? 0 : size_t(G) * dim * 16;
     static std::mutex mu;  // the device buffers below are shared by every handle
     std::lock_guard<std::mutex> lock(mu);
     static ZBuf b_apps, b_w, b_vals, b_amps, b_slots;
     static ZBuf b_ev[2], b_off[2];
     static hipEvent_t done[2] = {nullptr, nullptr};
     static int owner = -1;  // the device the cached buffers and events live on
+    static bool lds13 = false;  // the owner device gives a workgroup 128 KiB of LDS (one 13-qubit state)
     {
         int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_noisy_run: no device");
@@ -444,18 +461,36 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
                 if (e) (void)hipEventDestroy(e);
                 e = nullptr;
             }
+            // dynamic LDS: up to 64 KiB for the 512-thread variants (one 12-qubit state), 128 KiB
+            // for the 1024-thread ones where the device allows it (gfx950: 160 KiB per workgroup)
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zne<false, false, kZThreads>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zne<true, false, kZThreads>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
+            int optin = 0;
+            if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess) optin = 0;
+            lds13 = optin >= (128 << 10) + 1024 &&
+                    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zne<false, false, kZSlotThreads>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 128 << 10) == hipSuccess &&
+                    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zne<true, false, kZSlotThreads>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 128 << 10) == hipSuccess;
             owner = dev;
         }
     }
-    static bool attr_set = false;
-    if (!attr_set) {
-        // up to 64 KiB of dynamic LDS (one 12-qubit state)
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zne<false, false>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zne<true, false>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
-        attr_set = true;
-    }
+    // Launch shape.  Up to 12 qubits: 512-thread workgroups, L = min(512, 2^(n-1)) threads per
+    // trajectory, 512 / L states in LDS.  Above: one trajectory per 1024-thread workgroup, its state
+    // in LDS at 13 qubits where the device allows 128 KiB per workgroup (MI355X, 13 qubits: 83 gate
+    // applications per µs against 34 in slots), else in a device-memory slot.
+    const bool wide = n > kZLdsQubits;
+    const bool in_slot = wide && !(n == kZLdsQubits + 1 && lds13);
+    const int lgL = wide ? 10 : std::min(9, std::max(0, n - 1));
+    const int G = wide ? 1 : kZThreads >> lgL;
+    const size_t lds = in_slot ? 0 : size_t(G) * dim * 16;
+    using Launch = hipError_t (*)(unsigned, size_t, const ZArgs&);
+    const bool philox = z->rng != 0;
+    const Launch launch = !wide     ? (philox ? launch_zne<true, false, kZThreads> : launch_zne<false, false, kZThreads>)
+                          : in_slot ? (philox ? launch_zne<true, true, kZSlotThreads> : launch_zne<false, true, kZSlotThreads>)
+                                    : (philox ? launch_zne<true, false, kZSlotThreads> : launch_zne<false, false, kZSlotThreads>);
 #define ZK(expr)                                                                                         \
     do {                                                                                                 \
         const hipError_t e_ = (expr);                                                                    \
@@ -526,26 +561,15 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
         const double* dw = weights ? static_cast<const double*>(b_w.p) : nullptr;
         double* dvals = weights ? static_cast<double*>(b_vals.p) + t0 : nullptr;
         v2d* damps = amps ? static_cast<v2d*>(b_amps.p) : nullptr;
-        v2d* dslots = static_cast<v2d*>(b_slots.p);
-        const ZApp* dapps = static_cast<const ZApp*>(b_apps.p);
-        if (z->rng == 0) {
-            const uint2* dev_ev = static_cast<const uint2*>(b_ev[ev_slot].p);
-            const uint64_t* dev_off = static_cast<const uint64_t*>(b_off[ev_slot].p);
-            if (in_slot)
-                hipLaunchKernelGGL((k_zne<false, true>), dim3(grid), dim3(kZSlotThreads), 0, nullptr, dapps, S, n, lgL, nt,
-                                   dev_ev, dev_off, z->p, z->seed, uint64_t(0), dw, dvals, damps, dslots);
-            else
-                hipLaunchKernelGGL((k_zne<false, false>), dim3(grid), dim3(kZThreads), lds, nullptr, dapps, S, n, lgL, nt,
-                                   dev_ev, dev_off, z->p, z->seed, uint64_t(0), dw, dvals, damps, dslots);
+        ZArgs args{static_cast<const ZApp*>(b_apps.p), S, n, lgL, nt, nullptr, nullptr, z->p, z->seed, 0,
+                   dw, dvals, damps, static_cast<v2d*>(b_slots.p)};
+        if (philox) {
+            args.traj_base = z->philox_next + t0;
         } else {
-            if (in_slot)
-                hipLaunchKernelGGL((k_zne<true, true>), dim3(grid), dim3(kZSlotThreads), 0, nullptr, dapps, S, n, lgL, nt,
-                                   nullptr, nullptr, z->p, z->seed, z->philox_next + t0, dw, dvals, damps, dslots);
-            else
-                hipLaunchKernelGGL((k_zne<true, false>), dim3(grid), dim3(kZThreads), lds, nullptr, dapps, S, n, lgL, nt,
-                                   nullptr, nullptr, z->p, z->seed, z->philox_next + t0, dw, dvals, damps, dslots);
+            args.ev = static_cast<const uint2*>(b_ev[ev_slot].p);
+            args.ev_off = static_cast<const uint64_t*>(b_off[ev_slot].p);
         }
-        ZK(hipGetLastError());
+        ZK(launch(grid, lds, args));
         if (z->rng == 0) ZK(hipEventRecord(done[ev_slot], nullptr));
         if (amps) ZK(hipMemcpy(amps + 2 * t0 * dim, damps, size_t(nt * dim * 16), hipMemcpyDeviceToHost));
     }

@@ -212,7 +212,7 @@ def test_errors_and_size_limits():
                               lambda b: float(b.probabilities()[0]), [1, 3])
     o = Z.extrapolate(n, g21, 0.2, 4, 2, [1, 3], np.eye(1, 2 ** n, 0).ravel())
     assert abs(r.value - o["value"]) < 1e-12
-    # 13 qubits: the batched path with device-memory states
+    # 13 qubits: the batched path, one 1024-thread workgroup per trajectory
     n = 13
     nb13 = QM.NoisyBackend(QM.ReferenceBackend(n), 0.1, 4)
     gates = [("h", 0), ("cnot", 0, 12), ("ry", 5, 0.3)]
