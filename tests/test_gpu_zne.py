@@ -144,6 +144,23 @@ def test_philox_closed_forms_and_exact():
     ana = sum(c * z0 * r1 ** (3 * lam) for c, lam in zip(r.weights, r.scales))
     assert abs(r.value - ana) < 5 * r.std_error
     assert abs(r.value - z0) < abs(r.values[0] - z0)
+    # the same closed forms on wide registers: 13 qubits (state in LDS, 1024-thread workgroups) and
+    # 14 (state in a device-memory slot); <Z_0> after CIRC1 on qubit 0, <Z_0 Z_(n-1)> of a Bell pair
+    # across the register
+    r2w = 1.0 - 16.0 * 0.03 / 15.0
+    for n in (13, 14):
+        i = np.arange(2 ** n)
+        z_0 = 1.0 - 2.0 * (i & 1)
+        zz = z_0 * (1.0 - 2.0 * ((i >> (n - 1)) & 1))
+        nbw = QM.NoisyBackend(QM.ReferenceBackend(n), P1, 31 + n, rng="philox")
+        r = QM.ZNE(20000).extrapolate(nbw, lambda b: [_apply(b, g) for g in CIRC1], QM.Diagonal(z_0), [1, 3])
+        for lam, v, se in zip(r.scales, r.values, r.std_errors):
+            assert abs(v - z0 * r1 ** (3 * lam)) < 5 * se
+        nbw = QM.NoisyBackend(QM.ReferenceBackend(n), 0.03, 47 + n, rng="philox")
+        bell = [("h", 0), ("cnot", 0, n - 1)]
+        r = QM.ZNE(20000).extrapolate(nbw, lambda b: [_apply(b, g) for g in bell], QM.Diagonal(zz), [1, 3])
+        for lam, v, se in zip(r.scales, r.values, r.std_errors):
+            assert abs(v - r2w ** lam) < 5 * se
     # Bell pair, two-qubit channel: <ZZ>(lambda) = r2^lambda
     p2 = 0.03
     r2 = 1.0 - 16.0 * p2 / 15.0
@@ -212,14 +229,15 @@ def test_errors_and_size_limits():
                               lambda b: float(b.probabilities()[0]), [1, 3])
     o = Z.extrapolate(n, g21, 0.2, 4, 2, [1, 3], np.eye(1, 2 ** n, 0).ravel())
     assert abs(r.value - o["value"]) < 1e-12
-    # 13 qubits: the batched path, one 1024-thread workgroup per trajectory
-    n = 13
-    nb13 = QM.NoisyBackend(QM.ReferenceBackend(n), 0.1, 4)
-    gates = [("h", 0), ("cnot", 0, 12), ("ry", 5, 0.3)]
-    r = QM.ZNE(6).extrapolate(nb13, lambda b: [_apply(b, g) for g in gates],
-                              lambda b: float(b.probabilities()[0]), [1, 3])
-    o = Z.extrapolate(n, gates, 0.1, 4, 6, [1, 3], np.eye(1, 2 ** n, 0).ravel())
-    assert abs(r.value - o["value"]) < 1e-12
+    # 13 qubits (state in LDS) and 14 (state in a device-memory slot): the batched path, one
+    # 1024-thread workgroup per trajectory, final states copied out for a callable observable
+    for n in (13, 14):
+        nbn = QM.NoisyBackend(QM.ReferenceBackend(n), 0.1, 4)
+        gates = [("h", 0), ("cnot", 0, n - 1), ("ry", 5, 0.3)]
+        r = QM.ZNE(6).extrapolate(nbn, lambda b: [_apply(b, g) for g in gates],
+                                  lambda b: float(b.probabilities()[0]), [1, 3])
+        o = Z.extrapolate(n, gates, 0.1, 4, 6, [1, 3], np.eye(1, 2 ** n, 0).ravel())
+        assert abs(r.value - o["value"]) < 1e-12
 
 
 def test_every_site_errs_and_one_qubit():
