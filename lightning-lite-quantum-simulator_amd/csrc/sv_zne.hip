@@ -32,6 +32,8 @@ namespace {
 
 constexpr int kZThreads = 512;
 constexpr int kZSlotThreads = 1024;  // above 12 qubits: one trajectory per workgroup
+constexpr int kZTileQubits = 12;     // device-memory states: runs of applications on qubits below this
+constexpr int kZRunMax = 256;        // go through LDS tiles of 2^12 amplitudes, at most this many per run
 constexpr int kZLdsQubits = 12;  // up to 2^12 amplitudes (64 KiB) per state: 512-thread workgroups
 
 // One gate application of the folded circuit.
@@ -116,17 +118,91 @@ __device__ double group_sum(double v, int L, double* red) {
     return s;
 }
 
+// One gate application with its sampled Pauli (code) on a state of dim amplitudes at s (LDS or
+// device memory), by thread r of the L threads that work on it.  Every thread of the workgroup
+// calls it with the same application (a wider mcz has a barrier between its two passes); the
+// caller ends the pass with a barrier.
+__device__ __forceinline__ void apply_app(v2d* s, uint32_t dim, const ZApp& a, uint32_t code, uint32_t r,
+                                          uint32_t L) {
+    uint32_t xm, zm;
+    code_masks(a, code, xm, zm);
+    if (a.kind == 0 || (a.kind == 2 && a.nq == 1)) {
+        // pairs along q0: gate (or Z for a one-qubit mcz), then X^x Z^z on the pair
+        const int q = a.q0;
+        const bool zq = (zm >> q) & 1u, xq = (xm >> q) & 1u;
+        for (uint32_t pi = r; pi < dim / 2; pi += L) {
+            const uint32_t i0 = uint32_t(insert0(pi, q)), i1 = i0 | (1u << q);
+            v2d b0 = s[i0], b1 = s[i1];
+            if (a.kind == 0) {
+                const v2d c0 = row2(a.m, b0, b1), c1 = row2(a.m + 4, b0, b1);
+                b0 = c0;
+                b1 = c1;
+            } else {
+                b1 = -b1;
+            }
+            b1 = neg_if(b1, zq);
+            s[i0] = xq ? b1 : b0;
+            s[i1] = xq ? b0 : b1;
+        }
+    } else if (a.kind == 1 || (a.kind == 2 && a.nq == 2)) {
+        // quads over (q0, q1): cnot or cz, then X^x Z^z on the quad; k = bit q0 + 2 bit q1
+        const int qa = a.q0, qb = a.q1, lo = qa < qb ? qa : qb, hi = qa < qb ? qb : qa;
+        const uint32_t xk = ((xm >> qa) & 1u) | (((xm >> qb) & 1u) << 1);
+        const uint32_t zk = ((zm >> qa) & 1u) | (((zm >> qb) & 1u) << 1);
+        for (uint32_t qi = r; qi < dim / 4; qi += L) {
+            const uint32_t base = uint32_t(insert0(insert0(qi, lo), hi));
+            uint32_t idx[4];
+            v2d v[4];
+            for (int k = 0; k < 4; ++k) {
+                idx[k] = base | ((k & 1) ? (1u << qa) : 0u) | ((k & 2) ? (1u << qb) : 0u);
+                v[k] = s[idx[k]];
+            }
+            if (a.kind == 1) {  // control qa, target qb: |1,0> <-> |1,1>
+                const v2d tmp = v[1];
+                v[1] = v[3];
+                v[3] = tmp;
+            } else {
+                v[3] = -v[3];
+            }
+            for (int k = 0; k < 4; ++k) v[k] = neg_if(v[k], __builtin_popcount(uint32_t(k) & zk) & 1);
+            for (int k = 0; k < 4; ++k) s[idx[k]] = v[uint32_t(k) ^ xk];
+        }
+    } else {
+        // wider mcz: signs, then the per-qubit Paulis in a second pass
+        for (uint32_t i = r; i < dim; i += L)
+            if ((i & a.mask) == a.mask) s[i] = -s[i];
+        __syncthreads();
+        if (code) {
+            if (xm == 0) {
+                for (uint32_t i = r; i < dim; i += L) s[i] = neg_if(s[i], __builtin_popcount(i & zm) & 1);
+            } else {
+                const int hb = 31 - __builtin_clz(xm);
+                for (uint32_t pi = r; pi < dim / 2; pi += L) {
+                    const uint32_t i = uint32_t(insert0(pi, hb)), k = i ^ xm;
+                    const v2d vi = s[i], vk = s[k];
+                    s[i] = neg_if(vk, __builtin_popcount(k & zm) & 1);
+                    s[k] = neg_if(vi, __builtin_popcount(i & zm) & 1);
+                }
+            }
+        }
+    }
+}
+
 // NT threads per workgroup, 2^lgL of them per trajectory.  SLOT: the state is slot blockIdx.x of
-// `slots` (2^n amplitudes each) instead of LDS (then NT = 1024, lgL = 10: one trajectory per workgroup).
+// `slots` (2^n amplitudes each) instead of LDS (then NT = 1024, lgL = 10: one trajectory per
+// workgroup), and run[j] > 1 marks a run of applications j .. j + run[j] - 1 that act only on qubits
+// below kZTileQubits: the run is applied tile by tile (2^kZTileQubits consecutive amplitudes staged
+// in the dynamic LDS), so the slot is read and written once per run instead of once per application.
 template <bool PHILOX, bool SLOT, int NT>
 __global__ __launch_bounds__(NT) void k_zne(const ZApp* __restrict__ apps, int S, int n, int lgL,
-                                                   uint64_t n_traj, const uint2* __restrict__ ev,
-                                                   const uint64_t* __restrict__ ev_off, double p, uint64_t seed,
-                                                   uint64_t traj_base, const double* __restrict__ w,
-                                                   double* __restrict__ vals, v2d* __restrict__ amps,
-                                                   v2d* __restrict__ slots) {
+                                            uint64_t n_traj, const uint2* __restrict__ ev,
+                                            const uint64_t* __restrict__ ev_off, double p, uint64_t seed,
+                                            uint64_t traj_base, const double* __restrict__ w,
+                                            double* __restrict__ vals, v2d* __restrict__ amps,
+                                            v2d* __restrict__ slots, const int32_t* __restrict__ run) {
     extern __shared__ v2d lds_states[];
     __shared__ double red[NT / 64];
+    __shared__ uint32_t run_codes[SLOT ? kZRunMax : 1];
     const int L = 1 << lgL, G = NT >> lgL;
     const int g = int(threadIdx.x) >> lgL, r = int(threadIdx.x) & (L - 1);
     const uint32_t dim = 1u << n;
@@ -142,7 +218,40 @@ __global__ __launch_bounds__(NT) void k_zne(const ZApp* __restrict__ apps, int S
         }
         uint32_t next_site = cur < end ? ev[cur].x : 0xFFFFFFFFu;
         __syncthreads();
-        for (int j = 0; j < S; ++j) {
+        for (int j = 0; j < S;) {
+            if (SLOT && run[j] > 1) {
+                const int R = run[j];  // 2 .. kZRunMax
+                // the run's noise codes, by application
+                if (r < R)
+                    run_codes[r] = (PHILOX && live && p > 0.0)
+                                       ? philox_code(apps[j + r], p, seed, traj_base + t, uint32_t(j + r))
+                                       : 0u;
+                __syncthreads();
+                if (!PHILOX) {
+                    // every thread walks the event list (the same cursor everywhere); thread 0 writes
+                    while (live && next_site < uint32_t(j + R)) {
+                        if (r == 0) run_codes[next_site - uint32_t(j)] = ev[cur].y;
+                        ++cur;
+                        next_site = cur < end ? ev[cur].x : 0xFFFFFFFFu;
+                    }
+                    __syncthreads();
+                }
+                constexpr uint32_t kTile = 1u << kZTileQubits;
+                for (uint32_t k = 0; k < (dim >> kZTileQubits); ++k) {
+                    v2d* gt = s + size_t(k) * kTile;
+                    for (uint32_t i = uint32_t(r); i < kTile; i += uint32_t(L)) lds_states[i] = gt[i];
+                    __syncthreads();
+                    for (int u = 0; u < R; ++u) {
+                        apply_app(lds_states, kTile, apps[j + u], run_codes[u], uint32_t(r), uint32_t(L));
+                        __syncthreads();
+                    }
+                    // each thread stores the amplitudes it loaded, so the next tile's loads need no barrier
+                    for (uint32_t i = uint32_t(r); i < kTile; i += uint32_t(L)) gt[i] = lds_states[i];
+                }
+                __syncthreads();
+                j += R;
+                continue;
+            }
             const ZApp& a = apps[j];
             uint32_t code = 0;
             if (live) {
@@ -154,70 +263,9 @@ __global__ __launch_bounds__(NT) void k_zne(const ZApp* __restrict__ apps, int S
                     next_site = cur < end ? ev[cur].x : 0xFFFFFFFFu;
                 }
             }
-            uint32_t xm, zm;
-            code_masks(a, code, xm, zm);
-            if (a.kind == 0 || (a.kind == 2 && a.nq == 1)) {
-                // pairs along q0: gate (or Z for a one-qubit mcz), then X^x Z^z on the pair
-                const int q = a.q0;
-                const bool zq = (zm >> q) & 1u, xq = (xm >> q) & 1u;
-                for (uint32_t pi = uint32_t(r); pi < dim / 2; pi += uint32_t(L)) {
-                    const uint32_t i0 = uint32_t(insert0(pi, q)), i1 = i0 | (1u << q);
-                    v2d b0 = s[i0], b1 = s[i1];
-                    if (a.kind == 0) {
-                        const v2d c0 = row2(a.m, b0, b1), c1 = row2(a.m + 4, b0, b1);
-                        b0 = c0;
-                        b1 = c1;
-                    } else {
-                        b1 = -b1;
-                    }
-                    b1 = neg_if(b1, zq);
-                    s[i0] = xq ? b1 : b0;
-                    s[i1] = xq ? b0 : b1;
-                }
-            } else if (a.kind == 1 || (a.kind == 2 && a.nq == 2)) {
-                // quads over (q0, q1): cnot or cz, then X^x Z^z on the quad; k = bit q0 + 2 bit q1
-                const int qa = a.q0, qb = a.q1, lo = qa < qb ? qa : qb, hi = qa < qb ? qb : qa;
-                const uint32_t xk = ((xm >> qa) & 1u) | (((xm >> qb) & 1u) << 1);
-                const uint32_t zk = ((zm >> qa) & 1u) | (((zm >> qb) & 1u) << 1);
-                for (uint32_t qi = uint32_t(r); qi < dim / 4; qi += uint32_t(L)) {
-                    const uint32_t base = uint32_t(insert0(insert0(qi, lo), hi));
-                    uint32_t idx[4];
-                    v2d v[4];
-                    for (int k = 0; k < 4; ++k) {
-                        idx[k] = base | ((k & 1) ? (1u << qa) : 0u) | ((k & 2) ? (1u << qb) : 0u);
-                        v[k] = s[idx[k]];
-                    }
-                    if (a.kind == 1) {  // control qa, target qb: |1,0> <-> |1,1>
-                        const v2d tmp = v[1];
-                        v[1] = v[3];
-                        v[3] = tmp;
-                    } else {
-                        v[3] = -v[3];
-                    }
-                    for (int k = 0; k < 4; ++k) v[k] = neg_if(v[k], __builtin_popcount(uint32_t(k) & zk) & 1);
-                    for (int k = 0; k < 4; ++k) s[idx[k]] = v[uint32_t(k) ^ xk];
-                }
-            } else {
-                // wider mcz: signs, then the per-qubit Paulis in a second pass
-                for (uint32_t i = uint32_t(r); i < dim; i += uint32_t(L))
-                    if ((i & a.mask) == a.mask) s[i] = -s[i];
-                __syncthreads();
-                if (code) {
-                    if (xm == 0) {
-                        for (uint32_t i = uint32_t(r); i < dim; i += uint32_t(L))
-                            s[i] = neg_if(s[i], __builtin_popcount(i & zm) & 1);
-                    } else {
-                        const int hb = 31 - __builtin_clz(xm);
-                        for (uint32_t pi = uint32_t(r); pi < dim / 2; pi += uint32_t(L)) {
-                            const uint32_t i = uint32_t(insert0(pi, hb)), k = i ^ xm;
-                            const v2d vi = s[i], vk = s[k];
-                            s[i] = neg_if(vk, __builtin_popcount(k & zm) & 1);
-                            s[k] = neg_if(vi, __builtin_popcount(i & zm) & 1);
-                        }
-                    }
-                }
-            }
+            apply_app(s, dim, a, code, uint32_t(r), uint32_t(L));
             __syncthreads();
+            ++j;
         }
         if (w && vals) {
             double acc = 0.0;
@@ -247,12 +295,14 @@ struct ZArgs {
     double* vals;
     v2d* amps;
     v2d* slots;
+    const int32_t* run;
 };
 
 template <bool PHILOX, bool SLOT, int NT>
 hipError_t launch_zne(unsigned grid, size_t lds, const ZArgs& a) {
     hipLaunchKernelGGL((k_zne<PHILOX, SLOT, NT>), dim3(grid), dim3(NT), lds, nullptr, a.apps, a.S, a.n, a.lgL,
-                       a.n_traj, a.ev, a.ev_off, a.p, a.seed, a.traj_base, a.w, a.vals, a.amps, a.slots);
+                       a.n_traj, a.ev, a.ev_off, a.p, a.seed, a.traj_base, a.w, a.vals, a.amps, a.slots,
+                       a.run);
     return hipGetLastError();
 }
 
@@ -443,7 +493,7 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
 
     static std::mutex mu;  // the device buffers below are shared by every handle
     std::lock_guard<std::mutex> lock(mu);
-    static ZBuf b_apps, b_w, b_vals, b_amps, b_slots;
+    static ZBuf b_apps, b_w, b_vals, b_amps, b_slots, b_run;
     static ZBuf b_ev[2], b_off[2];
     static hipEvent_t done[2] = {nullptr, nullptr};
     static int owner = -1;  // the device the cached buffers and events live on
@@ -452,7 +502,8 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
         int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_noisy_run: no device");
         if (owner != dev) {  // another device is current: the buffers are allocated again there
-            for (ZBuf* b : {&b_apps, &b_w, &b_vals, &b_amps, &b_slots, &b_ev[0], &b_ev[1], &b_off[0], &b_off[1]}) {
+            for (ZBuf* b : {&b_apps, &b_w, &b_vals, &b_amps, &b_slots, &b_run, &b_ev[0], &b_ev[1], &b_off[0],
+                            &b_off[1]}) {
                 if (b->p) (void)hipFree(b->p);
                 b->p = nullptr;
                 b->cap = 0;
@@ -466,6 +517,11 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
             (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zne<false, false, kZThreads>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
             (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zne<true, false, kZThreads>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
+            // the device-memory variants stage 64 KiB tiles
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zne<false, true, kZSlotThreads>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zne<true, true, kZSlotThreads>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
             int optin = 0;
             if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess) optin = 0;
@@ -485,7 +541,7 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
     const bool in_slot = wide && !(n == kZLdsQubits + 1 && lds13);
     const int lgL = wide ? 10 : std::min(9, std::max(0, n - 1));
     const int G = wide ? 1 : kZThreads >> lgL;
-    const size_t lds = in_slot ? 0 : size_t(G) * dim * 16;
+    const size_t lds = in_slot ? size_t(16) << kZTileQubits : size_t(G) * dim * 16;
     using Launch = hipError_t (*)(unsigned, size_t, const ZArgs&);
     const bool philox = z->rng != 0;
     const Launch launch = !wide     ? (philox ? launch_zne<true, false, kZThreads> : launch_zne<false, false, kZThreads>)
@@ -523,6 +579,21 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
         const uint64_t cap = std::max<uint64_t>(1, (uint64_t(4) << 30) / slot_bytes);
         slot_blocks = unsigned(std::min<uint64_t>({want, cap, std::min(chunk, N)}));
         ZK(b_slots.need(size_t(slot_blocks) * slot_bytes));
+        // runs of at least two consecutive applications on qubits below kZTileQubits (run[j] at the
+        // run's first application; 1 elsewhere)
+        std::vector<int32_t> runs(size_t(S), 1);
+        for (int j = 0; j < S;) {
+            int k = j;
+            while (k < S && k - j < kZRunMax && apps[size_t(k)].mask < (1u << kZTileQubits)) ++k;
+            if (k - j >= 2) {
+                runs[size_t(j)] = k - j;
+                j = k;
+            } else {
+                ++j;
+            }
+        }
+        ZK(b_run.need(sizeof(int32_t) * runs.size()));
+        ZK(hipMemcpy(b_run.p, runs.data(), sizeof(int32_t) * runs.size(), hipMemcpyHostToDevice));
     }
     static PinBuf h_ev[2], h_off[2];
     for (int k = 0; k < 2; ++k)
@@ -562,7 +633,8 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
         double* dvals = weights ? static_cast<double*>(b_vals.p) + t0 : nullptr;
         v2d* damps = amps ? static_cast<v2d*>(b_amps.p) : nullptr;
         ZArgs args{static_cast<const ZApp*>(b_apps.p), S, n, lgL, nt, nullptr, nullptr, z->p, z->seed, 0,
-                   dw, dvals, damps, static_cast<v2d*>(b_slots.p)};
+                   dw, dvals, damps, static_cast<v2d*>(b_slots.p),
+                   in_slot ? static_cast<const int32_t*>(b_run.p) : nullptr};
         if (philox) {
             args.traj_base = z->philox_next + t0;
         } else {
