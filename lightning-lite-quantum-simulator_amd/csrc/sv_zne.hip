@@ -12,8 +12,10 @@
 // workgroup's own slot in device memory.  Per gate application: one pass over the gate's
 // amplitude pairs (one-qubit gates), quads (cnot, two-qubit mcz) or single amplitudes (wider
 // mcz); the sampled Pauli acts on the same qubits and is applied in the same pass (X^x Z^z on the
-// pair or quad), a workgroup barrier ends the pass.  A wider mcz takes a second pass for its per-qubit Paulis.  At the end each group
-// reduces sum_i w_i |psi_i|^2 for its trajectory and may copy its state out.
+// pair or quad), a workgroup barrier ends the pass.  A wider mcz takes a second pass for its
+// per-qubit Paulis.  In a device-memory slot, a run of consecutive applications on qubits 0-11 is
+// applied through 64 KiB LDS tiles instead, one read and one write of the slot per run.  At the
+// end each group reduces sum_i w_i |psi_i|^2 for its trajectory and may copy its state out.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -566,18 +568,16 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
     if (amps) chunk = std::max<uint64_t>(1, (uint64_t(256) << 20) / (dim * 16));
     if (z->rng == 0) chunk = std::min<uint64_t>(chunk, 2048);
     if (amps) ZK(b_amps.need(size_t(std::min(chunk, N) * dim * 16)));
-    // device-memory states (above 12 qubits): one slot per workgroup.  While half as many slots as
-    // CUs fit in 128 MiB (up to 16 qubits) that many run, so the slots stay in L2 / Infinity Cache
-    // (MI355X, 16 qubits: 7.7 TB/s of state traffic against 6.7 at two slots per CU); larger
-    // states stream from HBM and run one slot per CU (at most 4 GiB of slots).
+    // device-memory states: one slot per workgroup, one workgroup per CU, at most 4 GiB of slots
+    // (MI355X, with the LDS-tiled runs: 128 slots read 12.8 TB/s of state-equivalent traffic at 16
+    // qubits, 256 slots 19.2, 512 slots 18.3)
     unsigned slot_blocks = 0;
     if (in_slot) {
         int dev = 0, cus = 256;
         if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        const uint64_t half = std::max(1u, unsigned(cus) / 2), slot_bytes = dim * 16;
-        const uint64_t want = half * slot_bytes <= (uint64_t(128) << 20) ? half : uint64_t(unsigned(cus));
+        const uint64_t slot_bytes = dim * 16;
         const uint64_t cap = std::max<uint64_t>(1, (uint64_t(4) << 30) / slot_bytes);
-        slot_blocks = unsigned(std::min<uint64_t>({want, cap, std::min(chunk, N)}));
+        slot_blocks = unsigned(std::min<uint64_t>({uint64_t(std::max(1, cus)), cap, std::min(chunk, N)}));
         ZK(b_slots.need(size_t(slot_blocks) * slot_bytes));
         // runs of at least two consecutive applications on qubits below kZTileQubits (run[j] at the
         // run's first application; 1 elsewhere)
