@@ -34,7 +34,7 @@ namespace {
 
 constexpr int kZThreads = 512;
 constexpr int kZSlotThreads = 1024;  // above 12 qubits: one trajectory per workgroup
-constexpr int kZTileQubits = 12;     // device-memory states: runs of applications on qubits below this
+constexpr int kZTileQubits = 12;     // device-memory states: runs of applications on at most 12 qubits
 constexpr int kZRunMax = 256;        // go through LDS tiles of 2^12 amplitudes, at most this many per run
 constexpr int kZLdsQubits = 12;  // up to 2^12 amplitudes (64 KiB) per state: 512-thread workgroups
 
@@ -120,6 +120,14 @@ __device__ double group_sum(double v, int L, double* red) {
     return s;
 }
 
+// The bits of v deposited, lowest first, at the set bits of m.
+__device__ __forceinline__ uint32_t pdep(uint32_t v, uint32_t m) {
+    uint32_t out = 0;
+    for (uint32_t b = 1; m; b <<= 1, m &= m - 1)
+        if (v & b) out |= m & (0u - m);
+    return out;
+}
+
 // One gate application with its sampled Pauli (code) on a state of dim amplitudes at s (LDS or
 // device memory), by thread r of the L threads that work on it.  Every thread of the workgroup
 // calls it with the same application (a wider mcz has a barrier between its two passes); the
@@ -192,16 +200,19 @@ __device__ __forceinline__ void apply_app(v2d* s, uint32_t dim, const ZApp& a, u
 
 // NT threads per workgroup, 2^lgL of them per trajectory.  SLOT: the state is slot blockIdx.x of
 // `slots` (2^n amplitudes each) instead of LDS (then NT = 1024, lgL = 10: one trajectory per
-// workgroup), and run[j] > 1 marks a run of applications j .. j + run[j] - 1 that act only on qubits
-// below kZTileQubits: the run is applied tile by tile (2^kZTileQubits consecutive amplitudes staged
-// in the dynamic LDS), so the slot is read and written once per run instead of once per application.
+// workgroup), and run[j].x > 1 marks a run of applications j .. j + run[j].x - 1 whose qubits all lie
+// in the kZTileQubits-qubit set run[j].y: the run is applied tile by tile (the 2^kZTileQubits
+// amplitudes that share their bits outside the set, staged in the dynamic LDS; tapps holds the run's
+// applications with qubits renumbered by rank in the set), so the slot is read and written once per
+// run instead of once per application.
 template <bool PHILOX, bool SLOT, int NT>
 __global__ __launch_bounds__(NT) void k_zne(const ZApp* __restrict__ apps, int S, int n, int lgL,
                                             uint64_t n_traj, const uint2* __restrict__ ev,
                                             const uint64_t* __restrict__ ev_off, double p, uint64_t seed,
                                             uint64_t traj_base, const double* __restrict__ w,
                                             double* __restrict__ vals, v2d* __restrict__ amps,
-                                            v2d* __restrict__ slots, const int32_t* __restrict__ run) {
+                                            v2d* __restrict__ slots, const uint2* __restrict__ run,
+                                            const ZApp* __restrict__ tapps) {
     extern __shared__ v2d lds_states[];
     __shared__ double red[NT / 64];
     __shared__ uint32_t run_codes[SLOT ? kZRunMax : 1];
@@ -221,8 +232,8 @@ __global__ __launch_bounds__(NT) void k_zne(const ZApp* __restrict__ apps, int S
         uint32_t next_site = cur < end ? ev[cur].x : 0xFFFFFFFFu;
         __syncthreads();
         for (int j = 0; j < S;) {
-            if (SLOT && run[j] > 1) {
-                const int R = run[j];  // 2 .. kZRunMax
+            if (SLOT && run[j].x > 1) {
+                const int R = int(run[j].x);  // 2 .. kZRunMax
                 // the run's noise codes, by application
                 if (r < R)
                     run_codes[r] = (PHILOX && live && p > 0.0)
@@ -238,17 +249,22 @@ __global__ __launch_bounds__(NT) void k_zne(const ZApp* __restrict__ apps, int S
                     }
                     __syncthreads();
                 }
+                // tile element e = r + i L sits at slot offset pdep(e, T) + pdep(k, O) in tile k
                 constexpr uint32_t kTile = 1u << kZTileQubits;
+                constexpr int kPer = int(kTile) / kZSlotThreads;  // L = kZSlotThreads here
+                const uint32_t T = run[j].y, O = (dim - 1u) & ~T;
+                uint32_t off[kPer];
+                for (int i = 0; i < kPer; ++i) off[i] = pdep(uint32_t(r) + uint32_t(i) * uint32_t(L), T);
                 for (uint32_t k = 0; k < (dim >> kZTileQubits); ++k) {
-                    v2d* gt = s + size_t(k) * kTile;
-                    for (uint32_t i = uint32_t(r); i < kTile; i += uint32_t(L)) lds_states[i] = gt[i];
+                    v2d* gt = s + pdep(k, O);
+                    for (int i = 0; i < kPer; ++i) lds_states[uint32_t(r) + uint32_t(i) * uint32_t(L)] = gt[off[i]];
                     __syncthreads();
                     for (int u = 0; u < R; ++u) {
-                        apply_app(lds_states, kTile, apps[j + u], run_codes[u], uint32_t(r), uint32_t(L));
+                        apply_app(lds_states, kTile, tapps[j + u], run_codes[u], uint32_t(r), uint32_t(L));
                         __syncthreads();
                     }
                     // each thread stores the amplitudes it loaded, so the next tile's loads need no barrier
-                    for (uint32_t i = uint32_t(r); i < kTile; i += uint32_t(L)) gt[i] = lds_states[i];
+                    for (int i = 0; i < kPer; ++i) gt[off[i]] = lds_states[uint32_t(r) + uint32_t(i) * uint32_t(L)];
                 }
                 __syncthreads();
                 j += R;
@@ -297,14 +313,15 @@ struct ZArgs {
     double* vals;
     v2d* amps;
     v2d* slots;
-    const int32_t* run;
+    const uint2* run;
+    const ZApp* tapps;
 };
 
 template <bool PHILOX, bool SLOT, int NT>
 hipError_t launch_zne(unsigned grid, size_t lds, const ZArgs& a) {
     hipLaunchKernelGGL((k_zne<PHILOX, SLOT, NT>), dim3(grid), dim3(NT), lds, nullptr, a.apps, a.S, a.n, a.lgL,
                        a.n_traj, a.ev, a.ev_off, a.p, a.seed, a.traj_base, a.w, a.vals, a.amps, a.slots,
-                       a.run);
+                       a.run, a.tapps);
     return hipGetLastError();
 }
 
@@ -495,7 +512,7 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
 
     static std::mutex mu;  // the device buffers below are shared by every handle
     std::lock_guard<std::mutex> lock(mu);
-    static ZBuf b_apps, b_w, b_vals, b_amps, b_slots, b_run;
+    static ZBuf b_apps, b_w, b_vals, b_amps, b_slots, b_run, b_tapps;
     static ZBuf b_ev[2], b_off[2];
     static hipEvent_t done[2] = {nullptr, nullptr};
     static int owner = -1;  // the device the cached buffers and events live on
@@ -504,8 +521,8 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
         int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_noisy_run: no device");
         if (owner != dev) {  // another device is current: the buffers are allocated again there
-            for (ZBuf* b : {&b_apps, &b_w, &b_vals, &b_amps, &b_slots, &b_run, &b_ev[0], &b_ev[1], &b_off[0],
-                            &b_off[1]}) {
+            for (ZBuf* b : {&b_apps, &b_w, &b_vals, &b_amps, &b_slots, &b_run, &b_tapps, &b_ev[0], &b_ev[1],
+                            &b_off[0], &b_off[1]}) {
                 if (b->p) (void)hipFree(b->p);
                 b->p = nullptr;
                 b->cap = 0;
@@ -579,21 +596,41 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
         const uint64_t cap = std::max<uint64_t>(1, (uint64_t(4) << 30) / slot_bytes);
         slot_blocks = unsigned(std::min<uint64_t>({uint64_t(std::max(1, cus)), cap, std::min(chunk, N)}));
         ZK(b_slots.need(size_t(slot_blocks) * slot_bytes));
-        // runs of at least two consecutive applications on qubits below kZTileQubits (run[j] at the
-        // run's first application; 1 elsewhere)
-        std::vector<int32_t> runs(size_t(S), 1);
+        // runs: greedily the longest stretch of applications whose qubits together number at most
+        // kZTileQubits (and at most kZRunMax applications), if it holds two or more; the tile's qubit
+        // set is theirs, filled up with the lowest other qubits (whole 16-byte amplitudes, and runs of
+        // consecutive ones, per load)
+        std::vector<uint2> runs(size_t(S), make_uint2(1u, 0u));
+        std::vector<ZApp> tapps(apps);
         for (int j = 0; j < S;) {
+            uint32_t u = 0;
             int k = j;
-            while (k < S && k - j < kZRunMax && apps[size_t(k)].mask < (1u << kZTileQubits)) ++k;
-            if (k - j >= 2) {
-                runs[size_t(j)] = k - j;
-                j = k;
-            } else {
+            while (k < S && k - j < kZRunMax && __builtin_popcount(u | apps[size_t(k)].mask) <= kZTileQubits)
+                u |= apps[size_t(k++)].mask;
+            if (k - j < 2) {
                 ++j;
+                continue;
             }
+            uint32_t T = u;
+            for (int b = 0; __builtin_popcount(T) < kZTileQubits; ++b) T |= 1u << b;
+            runs[size_t(j)] = make_uint2(uint32_t(k - j), T);
+            const auto rank = [T](int q) { return int8_t(__builtin_popcount(T & ((1u << q) - 1u))); };
+            for (int x = j; x < k; ++x) {
+                ZApp& b = tapps[size_t(x)];
+                b.q0 = rank(b.q0);
+                b.q1 = rank(b.q1);
+                b.mask = 0;
+                for (int i = 0; i < b.nq; ++i) {
+                    b.ql[i] = rank(b.ql[i]);
+                    b.mask |= 1u << b.ql[i];
+                }
+            }
+            j = k;
         }
-        ZK(b_run.need(sizeof(int32_t) * runs.size()));
-        ZK(hipMemcpy(b_run.p, runs.data(), sizeof(int32_t) * runs.size(), hipMemcpyHostToDevice));
+        ZK(b_run.need(sizeof(uint2) * runs.size()));
+        ZK(hipMemcpy(b_run.p, runs.data(), sizeof(uint2) * runs.size(), hipMemcpyHostToDevice));
+        ZK(b_tapps.need(sizeof(ZApp) * tapps.size()));
+        ZK(hipMemcpy(b_tapps.p, tapps.data(), sizeof(ZApp) * tapps.size(), hipMemcpyHostToDevice));
     }
     static PinBuf h_ev[2], h_off[2];
     for (int k = 0; k < 2; ++k)
@@ -634,7 +671,8 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
         v2d* damps = amps ? static_cast<v2d*>(b_amps.p) : nullptr;
         ZArgs args{static_cast<const ZApp*>(b_apps.p), S, n, lgL, nt, nullptr, nullptr, z->p, z->seed, 0,
                    dw, dvals, damps, static_cast<v2d*>(b_slots.p),
-                   in_slot ? static_cast<const int32_t*>(b_run.p) : nullptr};
+                   in_slot ? static_cast<const uint2*>(b_run.p) : nullptr,
+                   in_slot ? static_cast<const ZApp*>(b_tapps.p) : nullptr};
         if (philox) {
             args.traj_base = z->philox_next + t0;
         } else {
