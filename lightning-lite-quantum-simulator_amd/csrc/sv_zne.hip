@@ -13,8 +13,8 @@
 // amplitude pairs (one-qubit gates), quads (cnot, two-qubit mcz) or single amplitudes (wider
 // mcz); the sampled Pauli acts on the same qubits and is applied in the same pass (X^x Z^z on the
 // pair or quad), a workgroup barrier ends the pass.  A wider mcz takes a second pass for its
-// per-qubit Paulis.  In a device-memory slot, a run of consecutive applications on qubits 0-11 is
-// applied through 64 KiB LDS tiles instead, one read and one write of the slot per run.  At the
+// per-qubit Paulis.  In a device-memory slot, a run of consecutive applications on at most 12
+// qubits is applied through 64 KiB LDS tiles instead, one read and one write of the slot per run.  At the
 // end each group reduces sum_i w_i |psi_i|^2 for its trajectory and may copy its state out.
 #include <hip/hip_runtime.h>
 
