@@ -55,12 +55,13 @@ def gpu_values(n, gates, p, seed, scale, n_traj, w, rng="reference", noisy=None)
 
 @pytest.mark.parametrize("n,G,p,seed", [(1, 6, 0.3, 1), (2, 10, 0.2, 2), (3, 15, 0.2, 3), (5, 30, 0.1, 4),
                                         (8, 40, 0.05, 5), (10, 30, 0.05, 6), (12, 24, 0.04, 7),
-                                        (13, 20, 0.04, 8), (15, 14, 0.03, 9)])
+                                        (13, 20, 0.04, 8), (15, 14, 0.03, 9), (17, 12, 0.2, 27),
+                                        (20, 8, 0.2, 30)])
 def test_reference_stream_matches_oracle(n, G, p, seed):
     # every trajectory: the oracle's Pauli insertions (same mt19937_64 stream), the same state
     gates = random_circuit(n, G, seed)
     w = np.random.default_rng(seed).uniform(-1, 1, 2 ** n)
-    ntr = 40 if n >= 10 else 120
+    ntr = 6 if n >= 17 else 40 if n >= 10 else 120  # the oracle takes about 5 s at 20 qubits
     for scale in (1, 3):
         say(f"zne n={n} G={G} p={p} seed={seed} scale={scale}")
         ours = gpu_values(n, gates, p, seed, scale, ntr, w)
