@@ -326,7 +326,7 @@ int llsv_traj_default_config(int threads, int per_cu, int fused);
 #define LLSV_ZG_RZ 4
 #define LLSV_ZG_CNOT 5 /* q[0] = control, q[1] = target */
 #define LLSV_ZG_MCZ 6  /* q[0..nq) */
-#define LLSV_ZNE_MAX_QUBITS 20 /* up to 13 qubits a trajectory's state lives in LDS, above in a device-memory slot */
+#define LLSV_ZNE_MAX_QUBITS 20 /* up to 12 qubits a trajectory's state lives in LDS, above in a device-memory slot */
 
 typedef struct llsv_zgate {
     int32_t kind; /* LLSV_ZG_* */
@@ -357,7 +357,7 @@ int llsv_noisy_reseed(llsv_noisy* z, uint64_t seed);
 int llsv_noisy_site(llsv_noisy* z, int nq, uint32_t* code);
 
 /* Runs n_traj trajectories of the logical circuit `gates` at noise scale `scale` (odd, 1..99)
- * from |0...0> on n <= LLSV_ZNE_MAX_QUBITS qubits (a state of up to 13 qubits lives in a
+ * from |0...0> on n <= LLSV_ZNE_MAX_QUBITS qubits (a state of up to 12 qubits lives in a
  * workgroup's LDS, a larger one in a device-memory slot of the library: at most 4 GiB of slots, kept
  * on the device for later calls until the process ends).  weights (host, 2^n doubles, or NULL): values[t]
  * = sum_i weights[i] |psi_t[i]|^2 (host, n_traj doubles).  amps (host, n_traj * 2^n complex128

@@ -7,15 +7,14 @@
 // from counter-based Philox draws made on the device.  Nothing is normalised and nothing waits on
 // a probability, so a trajectory is a plain gate sweep: each one lives in LDS (2^n x 16 B, n <= 12)
 // and owns L = min(512, 2^(n-1)) threads of a 512-thread workgroup (512 / L trajectories per
-// workgroup); above 12 qubits (up to 20) one trajectory per 1024-thread workgroup, its state in
-// 128 KiB of LDS at 13 qubits (where the device allows that much per workgroup), else the
+// workgroup); above 12 qubits (up to 20) one trajectory per 1024-thread workgroup, its state in the
 // workgroup's own slot in device memory.  Per gate application: one pass over the gate's
 // amplitude pairs (one-qubit gates), quads (cnot, two-qubit mcz) or single amplitudes (wider
 // mcz); the sampled Pauli acts on the same qubits and is applied in the same pass (X^x Z^z on the
 // pair or quad), a workgroup barrier ends the pass.  A wider mcz takes a second pass for its
 // per-qubit Paulis.  In a device-memory slot, a run of consecutive applications on at most 12
-// qubits is applied through 64 KiB LDS tiles instead, one read and one write of the slot per run.  At the
-// end each group reduces sum_i w_i |psi_i|^2 for its trajectory and may copy its state out.
+// qubits is applied through 64 KiB LDS tiles instead, one read and one write of the slot per run.
+// At the end each group reduces sum_i w_i |psi_i|^2 for its trajectory and may copy its state out.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -516,7 +515,6 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
     static ZBuf b_ev[2], b_off[2];
     static hipEvent_t done[2] = {nullptr, nullptr};
     static int owner = -1;  // the device the cached buffers and events live on
-    static bool lds13 = false;  // the owner device gives a workgroup 128 KiB of LDS (one 13-qubit state)
     {
         int dev = 0;
         if (hipGetDevice(&dev) != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_noisy_run: no device");
@@ -531,41 +529,30 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
                 if (e) (void)hipEventDestroy(e);
                 e = nullptr;
             }
-            // dynamic LDS: up to 64 KiB for the 512-thread variants (one 12-qubit state), 128 KiB
-            // for the 1024-thread ones where the device allows it (gfx950: 160 KiB per workgroup)
+            // 64 KiB of dynamic LDS: one 12-qubit state, or one tile of a device-memory state
             (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zne<false, false, kZThreads>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
             (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zne<true, false, kZThreads>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
-            // the device-memory variants stage 64 KiB tiles
             (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zne<false, true, kZSlotThreads>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
             (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zne<true, true, kZSlotThreads>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 65536);
-            int optin = 0;
-            if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess) optin = 0;
-            lds13 = optin >= (128 << 10) + 1024 &&
-                    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zne<false, false, kZSlotThreads>),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, 128 << 10) == hipSuccess &&
-                    hipFuncSetAttribute(reinterpret_cast<const void*>(&k_zne<true, false, kZSlotThreads>),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, 128 << 10) == hipSuccess;
             owner = dev;
         }
     }
     // Launch shape.  Up to 12 qubits: 512-thread workgroups, L = min(512, 2^(n-1)) threads per
-    // trajectory, 512 / L states in LDS.  Above: one trajectory per 1024-thread workgroup, its state
-    // in LDS at 13 qubits where the device allows 128 KiB per workgroup (MI355X, 13 qubits: 83 gate
-    // applications per µs against 34 in slots), else in a device-memory slot.
-    const bool wide = n > kZLdsQubits;
-    const bool in_slot = wide && !(n == kZLdsQubits + 1 && lds13);
-    const int lgL = wide ? 10 : std::min(9, std::max(0, n - 1));
-    const int G = wide ? 1 : kZThreads >> lgL;
+    // trajectory, 512 / L states in LDS.  Above: one trajectory per 1024-thread workgroup, its state in
+    // a device-memory slot with LDS-tiled runs (MI355X, 13 qubits: 107 gate applications per µs against
+    // 85 with the whole 128 KiB state in LDS, one workgroup per CU).
+    const bool in_slot = n > kZLdsQubits;
+    const int lgL = in_slot ? 10 : std::min(9, std::max(0, n - 1));
+    const int G = in_slot ? 1 : kZThreads >> lgL;
     const size_t lds = in_slot ? size_t(16) << kZTileQubits : size_t(G) * dim * 16;
     using Launch = hipError_t (*)(unsigned, size_t, const ZArgs&);
     const bool philox = z->rng != 0;
-    const Launch launch = !wide     ? (philox ? launch_zne<true, false, kZThreads> : launch_zne<false, false, kZThreads>)
-                          : in_slot ? (philox ? launch_zne<true, true, kZSlotThreads> : launch_zne<false, true, kZSlotThreads>)
-                                    : (philox ? launch_zne<true, false, kZSlotThreads> : launch_zne<false, false, kZSlotThreads>);
+    const Launch launch = in_slot ? (philox ? launch_zne<true, true, kZSlotThreads> : launch_zne<false, true, kZSlotThreads>)
+                                  : (philox ? launch_zne<true, false, kZThreads> : launch_zne<false, false, kZThreads>);
 #define ZK(expr)                                                                                         \
     do {                                                                                                 \
         const hipError_t e_ = (expr);                                                                    \

@@ -9,7 +9,7 @@ src/Mitigation.cpp).
   whether gates are applied one at a time or trajectories run in a batch.
 * `ZNE(trajectories).extrapolate(noisy, circuit_fn, observable_fn, scales)` (src/Mitigation.h:
   340-390): for states of up to 20 qubits the circuit is recorded once and every scale's
-  trajectories run in one GPU launch (llsv_noisy_run), each trajectory's state in LDS (up to 13
+  trajectories run in one GPU launch (llsv_noisy_run), each trajectory's state in LDS (up to 12
   qubits) or in a per-workgroup device-memory slot; a `Diagonal` observable is evaluated in the
   kernel, any other callable on each final state.
   Larger states run trajectory by trajectory through `noisy` exactly like the reference's loop.

@@ -145,9 +145,8 @@ def test_philox_closed_forms_and_exact():
     ana = sum(c * z0 * r1 ** (3 * lam) for c, lam in zip(r.weights, r.scales))
     assert abs(r.value - ana) < 5 * r.std_error
     assert abs(r.value - z0) < abs(r.values[0] - z0)
-    # the same closed forms on wide registers: 13 qubits (state in LDS, 1024-thread workgroups) and
-    # 14 (state in a device-memory slot); <Z_0> after CIRC1 on qubit 0, <Z_0 Z_(n-1)> of a Bell pair
-    # across the register
+    # the same closed forms on wide registers (13 and 14 qubits, states in device-memory slots):
+    # <Z_0> after CIRC1 on qubit 0, <Z_0 Z_(n-1)> of a Bell pair across the register
     r2w = 1.0 - 16.0 * 0.03 / 15.0
     for n in (13, 14):
         i = np.arange(2 ** n)
@@ -230,8 +229,8 @@ def test_errors_and_size_limits():
                               lambda b: float(b.probabilities()[0]), [1, 3])
     o = Z.extrapolate(n, g21, 0.2, 4, 2, [1, 3], np.eye(1, 2 ** n, 0).ravel())
     assert abs(r.value - o["value"]) < 1e-12
-    # 13 qubits (state in LDS) and 14 (state in a device-memory slot): the batched path, one
-    # 1024-thread workgroup per trajectory, final states copied out for a callable observable
+    # 13 and 14 qubits: the batched path, one 1024-thread workgroup per trajectory and its state in a
+    # device-memory slot, final states copied out for a callable observable
     for n in (13, 14):
         nbn = QM.NoisyBackend(QM.ReferenceBackend(n), 0.1, 4)
         gates = [("h", 0), ("cnot", 0, n - 1), ("ry", 5, 0.3)]
