@@ -357,9 +357,10 @@ int llsv_noisy_reseed(llsv_noisy* z, uint64_t seed);
 int llsv_noisy_site(llsv_noisy* z, int nq, uint32_t* code);
 
 /* Runs n_traj trajectories of the logical circuit `gates` at noise scale `scale` (odd, 1..99)
- * from |0...0> on n <= LLSV_ZNE_MAX_QUBITS qubits (a state of up to 12 qubits lives in a
- * workgroup's LDS, a larger one in a device-memory slot of the library: at most 4 GiB of slots, kept
- * on the device for later calls until the process ends).  weights (host, 2^n doubles, or NULL): values[t]
+ * from |0...0> on n <= LLSV_ZNE_MAX_QUBITS qubits, n_gates * scale <= 2^22 gate applications (a
+ * state of up to 12 qubits lives in a workgroup's LDS, a larger one in a device-memory slot of the
+ * library: at most 4 GiB of slots, kept on the device for later calls until the process ends).
+ * weights (host, 2^n doubles, or NULL): values[t]
  * = sum_i weights[i] |psi_t[i]|^2 (host, n_traj doubles).  amps (host, n_traj * 2^n complex128
  * interleaved, or NULL): the final states.  With p = 0 no draws are made.  Replaces the
  * trajectory loop of ZNE::extrapolate (src/Mitigation.h:355-376) for one scale. */

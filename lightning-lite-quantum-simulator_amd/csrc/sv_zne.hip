@@ -35,6 +35,7 @@ constexpr int kZThreads = 512;
 constexpr int kZSlotThreads = 1024;  // above 12 qubits: one trajectory per workgroup
 constexpr int kZTileQubits = 12;     // device-memory states: runs of applications on at most 12 qubits
 constexpr int kZRunMax = 256;        // go through LDS tiles of 2^12 amplitudes, at most this many per run
+constexpr int64_t kZMaxApps = int64_t(1) << 22;  // gate applications of one folded circuit
 constexpr int kZLdsQubits = 12;  // up to 2^12 amplitudes (64 KiB) per state: 512-thread workgroups
 
 // One gate application of the folded circuit.
@@ -502,6 +503,9 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
     if (n_traj < 0) return fail(LLSV_ERR_ARG, "llsv_noisy_run: n_traj < 0");
     if ((weights == nullptr) != (values == nullptr))
         return fail(LLSV_ERR_ARG, "llsv_noisy_run: weights and values go together");
+    if (int64_t(n_gates) * scale > kZMaxApps)
+        return fail(LLSV_ERR_ARG, "llsv_noisy_run: %d gates at scale %d exceed %lld gate applications", n_gates, scale,
+                    static_cast<long long>(kZMaxApps));
     if (int rc = check_gates(n, gates, n_gates)) return rc;
     if (n_traj == 0) return LLSV_OK;
 

@@ -296,6 +296,9 @@ def test_product_host_validation_and_weights():
             QM.ZNE.validate_scales(sc)
     with pytest.raises(ValueError):
         QM.ZNE(0)
+    # the folded circuit is bounded (2^22 gate applications), checked before any device work
+    with pytest.raises(ValueError, match="gate applications"):
+        QM.run_trajectories(nb, 1, [("h", 0)] * 42400, 99, 1, QM.Diagonal(np.ones(2)))
     with pytest.raises(ValueError):
         QM.ZNE.richardson([1.0, 3.0], [1.0])
     with pytest.raises(ValueError):
