@@ -14,6 +14,10 @@ roofline  = the dominant kernel (the one-qubit gate): 32 * 2^n bytes / its avera
             one-qubit launch inside the timed region; peak 8000 GB/s.
 cpu_baseline = oracle/cpu_baseline.py in a subprocess (reference v3 loop on host cores).
 
+The same line carries the other paths' measurements (skipped with --no-extra; --leg NAME runs one
+alone and prints only it): latency12, layered28, adjoint20, trajectories14, zne12, zne16,
+lindblad10, q32 (program28 only through --leg).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--qubits 30] [--no-cpu-baseline]
 """
 from __future__ import annotations
