@@ -69,6 +69,20 @@ def test_reference_stream_matches_oracle(n, G, p, seed):
         assert np.max(np.abs(ours - theirs)) < 1e-12
 
 
+def test_wide_mcz_between_tiled_runs():
+    # 15 qubits (device-memory states): mcz gates on 14 qubits are wider than a 12-qubit tile, so they
+    # split the tiled runs and take single passes, with a noise channel on each of their qubits
+    n, p, seed = 15, 0.3, 21
+    gates = ([("h", q) for q in range(n)] + [("mcz", list(range(14)))] + [("ry", 14, 0.3), ("cnot", 14, 2)]
+             + [("mcz", list(range(1, 15)))] + [("rx", q, 0.1 * q) for q in range(6)])
+    w = np.random.default_rng(seed).uniform(-1, 1, 2 ** n)
+    for scale in (1, 3):
+        say(f"zne wide mcz n={n} scale={scale}")
+        ours = gpu_values(n, gates, p, seed, scale, 12, w)
+        theirs = Z.noisy_values(n, gates, p, seed, scale, 12, w)
+        assert np.max(np.abs(ours - theirs)) < 1e-12
+
+
 def test_reference_stream_matches_compiled_reference():
     lib = ref_lib()
     if lib is None:
