@@ -3,6 +3,8 @@
 Per trajectory the jump decisions must be identical (same Philox streams: equal jump counts and
 decision hashes) and the final <Z_q> within 1e-12; the ensemble must match the density matrix
 within 4.5 standard errors."""
+import json
+import os
 import sys
 
 import numpy as np
@@ -55,6 +57,31 @@ def test_ensemble_vs_density_matrix():
     zdm = T.density_matrix_z(n, as_u(gates), gamma, lam)
     se = z[:, :n].std(0) / np.sqrt(len(z)) + 1e-12
     assert np.all(np.abs(z[:, :n].mean(0) - zdm) < 4.5 * se)
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_ensemble_matches_density_matrix_fixture(case):
+    # the trajectory config at full size: 4096 trajectories of the 14-qubit brick circuit (and its
+    # 10-qubit sibling), ensemble <Z_q> within 3 standard errors of the exact density matrix
+    # (tests/golden/dm_brick14.json, written by the block update of src/Noise.cpp:174-258)
+    from llsv import trajectory as LT
+
+    with open(os.path.join(GOLDEN, "dm_brick14.json")) as f:
+        fx = json.load(f)["cases"][case]
+    n, count = fx["n_qubits"], 4096
+    gates, gamma, lam = C.noisy_brick(fx["circuit_seed"], n, fx["layers"])
+    assert len(gates) == fx["n_gates"] and np.array_equal(gamma, np.array(fx["gamma"]))
+    say(f"ensemble {count} x {n} q vs density matrix")
+    z, j, _ = LT.run_trajectories(n, gates, gamma, lam, seed=1, n_traj=count)
+    zt = z[:, :n]
+    se = zt.std(0, ddof=1) / np.sqrt(count)
+    dev = np.abs(zt.mean(0) - np.array(fx["z"])) / se
+    say(f"max deviation {dev.max():.2f} sigma, jumps {int(j.sum())}")
+    assert j.sum() > 0 and np.all(se > 0)
+    assert np.all(dev < 3.0), dev
 
 
 def test_mixed_batch_one_launch():
