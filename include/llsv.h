@@ -439,6 +439,16 @@ int llsv_mark_elapsed(llsv_state* s, int a, int b, float* ms);
 /* Wait for all work queued on `stream` (a hipStream_t; NULL = the default stream). */
 int llsv_stream_sync(void* stream);
 
+/* ---- testing ------------------------------------------------------------------ */
+
+/* Makes one device or pinned-host allocation of the library fail as if memory had run out: the
+ * one `after` allocations from now (0 = the next one); the hook then disarms.  -1 disarms it.
+ * Returns the previous setting: the allocations that were still to pass before the armed
+ * failure, or -1 if none was armed (so a caller can tell whether the failure was used).
+ * Process-wide.  For tests of the LLSV_ERR_NOMEM paths (state creation, program buffers,
+ * staging, readout, trajectory, ZNE and Lindblad workspaces); no counterpart in the reference. */
+long llsv_test_fail_allocation(long after);
+
 /* Message of the last failure on this thread ("" if none). */
 const char* llsv_last_error(void);
 

@@ -647,7 +647,7 @@ int run(int n, int B, const llsv_pgate* gates, int G, int P, const double* param
 #define CK(expr)                                                                                       \
     do {                                                                                               \
         const hipError_t e_ = (expr);                                                                  \
-        if (e_ != hipSuccess) return fail(LLSV_ERR_DEVICE, "%s: %s: %s", who, #expr, hipGetErrorString(e_)); \
+        if (e_ != hipSuccess) return fail(hip_code(e_), "%s: %s: %s", who, #expr, hipGetErrorString(e_)); \
     } while (0)
 
     std::vector<PauliDev> ht(T);

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -26,6 +27,32 @@ namespace llsv {
 
 static thread_local std::string g_last_error;
 
+// Allocations left before the injected failure (-1: none armed).
+static std::atomic<long> g_fail_after{-1};
+
+static bool injected_failure() {
+    long v = g_fail_after.load();
+    while (v >= 0)
+        if (g_fail_after.compare_exchange_weak(v, v - 1)) return v == 0;
+    return false;
+}
+
+hipError_t dev_alloc(void** p, size_t bytes) {
+    if (injected_failure()) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc(p, bytes);
+}
+
+hipError_t dev_host_alloc(void** p, size_t bytes) {
+    if (injected_failure()) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipHostMalloc(p, bytes, hipHostMallocDefault);
+}
+
 void set_error(const std::string& msg) { g_last_error = msg; }
 
 int fail(int code, const char* fmt, ...) {
@@ -41,7 +68,7 @@ int fail(int code, const char* fmt, ...) {
 #define LLSV_HIP(expr)                                                                                  \
     do {                                                                                                \
         const hipError_t e_ = (expr);                                                                   \
-        if (e_ != hipSuccess) return fail(LLSV_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+        if (e_ != hipSuccess) return fail(hip_code(e_), "%s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
 #define LLSV_HANDLE(s) \
@@ -53,6 +80,10 @@ static int check_qubit(const llsv_state* s, int q, const char* who) {
 }
 
 static constexpr size_t kParamBytes = 1 << 20;
+static constexpr uint64_t kProbChunk = uint64_t(1) << 20;  // doubles per llsv_probabilities step (8 MiB)
+static constexpr int kStageSlots = 8;
+// one llsv_apply_dense upload: the 32 x 32 complex matrix and 32 offsets at most
+static constexpr size_t kStageBytes = 32 * 32 * 16 + 32 * 8;
 
 static void free_state(llsv_state* s) {
     if (!s) return;
@@ -62,6 +93,10 @@ static void free_state(llsv_state* s) {
     if (s->partials) (void)hipFree(s->partials);
     if (s->result_dev) (void)hipFree(s->result_dev);
     if (s->params) (void)hipFree(s->params);
+    if (s->prob_chunk) (void)hipFree(s->prob_chunk);
+    if (s->stage) (void)hipHostFree(s->stage);
+    for (hipEvent_t& e : s->stage_ev)
+        if (e) (void)hipEventDestroy(e);
     if (s->result_host) (void)hipHostFree(s->result_host);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -86,17 +121,17 @@ static int alloc_state(int n, llsv_state** out) {
         free_state(s);
         return fail(LLSV_ERR_DEVICE, "llsv_create: %s", hipGetErrorString(e));
     }
-    e = hipMalloc(reinterpret_cast<void**>(&s->amps), s->size * sizeof(v2d));
+    e = dev_alloc(reinterpret_cast<void**>(&s->amps), s->size * sizeof(v2d));
     if (e != hipSuccess) {
         free_state(s);
         (void)hipGetLastError();
         return fail(LLSV_ERR_NOMEM, "llsv_create: cannot allocate %llu bytes of device memory for %d qubits",
                     static_cast<unsigned long long>(uint64_t(1) << n) * 16ull, n);
     }
-    if ((e = hipMalloc(reinterpret_cast<void**>(&s->partials), 2 * kRedBlocks * sizeof(double))) != hipSuccess ||
-        (e = hipMalloc(reinterpret_cast<void**>(&s->result_dev), 4 * sizeof(double))) != hipSuccess ||
-        (e = hipMalloc(reinterpret_cast<void**>(&s->params), kParamBytes)) != hipSuccess ||
-        (e = hipHostMalloc(reinterpret_cast<void**>(&s->result_host), 4 * sizeof(double))) != hipSuccess ||
+    if ((e = dev_alloc(reinterpret_cast<void**>(&s->partials), 2 * kRedBlocks * sizeof(double))) != hipSuccess ||
+        (e = dev_alloc(reinterpret_cast<void**>(&s->result_dev), 4 * sizeof(double))) != hipSuccess ||
+        (e = dev_alloc(reinterpret_cast<void**>(&s->params), kParamBytes)) != hipSuccess ||
+        (e = dev_host_alloc(reinterpret_cast<void**>(&s->result_host), 4 * sizeof(double))) != hipSuccess ||
         (e = hipEventCreate(&s->ev0)) != hipSuccess || (e = hipEventCreate(&s->ev1)) != hipSuccess) {
         free_state(s);
         (void)hipGetLastError();
@@ -121,6 +156,8 @@ extern "C" {
 const char* llsv_last_error(void) { return g_last_error.c_str(); }
 
 const char* llsv_version(void) { return "llsv 0.1 gfx950"; }
+
+long llsv_test_fail_allocation(long after) { return g_fail_after.exchange(after < 0 ? -1 : after); }
 
 int llsv_create(int n_qubits, llsv_state** out) {
     int rc = alloc_state(n_qubits, out);
@@ -231,8 +268,28 @@ int llsv_apply_dense(llsv_state* s, int k, const int* targets, const double* m) 
         off[c] = o;
     }
     const size_t ubytes = size_t(D) * D * sizeof(v2d);
-    LLSV_HIP(hipMemcpyAsync(s->params, m, ubytes, hipMemcpyHostToDevice, s->stream));
-    LLSV_HIP(hipMemcpyAsync(s->params + ubytes, off, D * sizeof(uint64_t), hipMemcpyHostToDevice, s->stream));
+    // Upload through a pinned slot: the copy reads it after this call returns, so a slot is
+    // rewritten only once the copy that last read it has completed.
+    if (!s->stage) {
+        if (dev_host_alloc(reinterpret_cast<void**>(&s->stage), kStageSlots * kStageBytes) != hipSuccess) {
+            s->stage = nullptr;
+            (void)hipGetLastError();
+            return fail(LLSV_ERR_NOMEM, "llsv_apply_dense: cannot allocate %zu bytes of pinned staging",
+                        kStageSlots * kStageBytes);
+        }
+    }
+    const int slot = s->stage_next;
+    s->stage_next = (slot + 1) % kStageSlots;
+    hipEvent_t& ev = s->stage_ev[slot];
+    if (ev)
+        LLSV_HIP(hipEventSynchronize(ev));
+    else
+        LLSV_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    unsigned char* stage = s->stage + size_t(slot) * kStageBytes;
+    std::memcpy(stage, m, ubytes);
+    std::memcpy(stage + ubytes, off, D * sizeof(uint64_t));
+    LLSV_HIP(hipMemcpyAsync(s->params, stage, ubytes + D * sizeof(uint64_t), hipMemcpyHostToDevice, s->stream));
+    LLSV_HIP(hipEventRecord(ev, s->stream));
     PassDesc pd;
     DenseTile dt;
     if (make_dense_tile(s->n, k, targets, &pd, &dt))
@@ -276,15 +333,22 @@ int llsv_probabilities(llsv_state* s, double* out, uint64_t n) {
     if (!out) return fail(LLSV_ERR_ARG, "llsv_probabilities: out is null");
     if (n != s->size) return fail(LLSV_ERR_ARG, "llsv_probabilities: buffer holds %llu values, state has %llu",
                                   (unsigned long long)n, (unsigned long long)s->size);
-    double* tmp = nullptr;
-    if (hipMalloc(reinterpret_cast<void**>(&tmp), n * sizeof(double)) != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(LLSV_ERR_NOMEM, "llsv_probabilities: cannot allocate %llu bytes", (unsigned long long)n * 8);
+    // |psi|^2 in chunks through one device buffer kept by the state (no allocation per call)
+    if (!s->prob_chunk) {
+        if (dev_alloc(reinterpret_cast<void**>(&s->prob_chunk), kProbChunk * sizeof(double)) != hipSuccess) {
+            s->prob_chunk = nullptr;
+            (void)hipGetLastError();
+            return fail(LLSV_ERR_NOMEM, "llsv_probabilities: cannot allocate %llu bytes",
+                        (unsigned long long)(kProbChunk * sizeof(double)));
+        }
     }
-    hipError_t e = launch_probabilities(s->stream, s->amps, s->n, tmp);
-    if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, n * sizeof(double), hipMemcpyDeviceToHost, s->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    (void)hipFree(tmp);
+    hipError_t e = hipSuccess;
+    for (uint64_t off = 0; off < n && e == hipSuccess; off += kProbChunk) {
+        const uint64_t cnt = std::min(kProbChunk, n - off);
+        e = launch_probabilities(s->stream, s->amps + off, cnt, s->prob_chunk);
+        if (e == hipSuccess) e = hipMemcpyAsync(out + off, s->prob_chunk, cnt * sizeof(double), hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    }
     if (e != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_probabilities: %s", hipGetErrorString(e));
     return LLSV_OK;
 }

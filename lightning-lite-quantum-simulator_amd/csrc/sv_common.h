@@ -6,6 +6,8 @@
 #include <cstdint>
 #include <string>
 
+#include "../../include/llsv.h"
+
 namespace llsv {
 
 // One complex128 amplitude as a 16-byte vector: x = re, y = im.  Loads and stores of
@@ -47,6 +49,13 @@ inline unsigned grid_for(uint64_t chunks) {
 
 __device__ __forceinline__ v2d ld_nt(const v2d* p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ void st_nt(v2d* p, v2d v) { __builtin_nontemporal_store(v, p); }
+
+// Every device and pinned-host allocation of the library goes through these two, so a test can
+// make the k-th one fail (llsv_test_fail_allocation) and reach each out-of-memory path.
+hipError_t dev_alloc(void** p, size_t bytes);
+// LLSV_ERR_NOMEM for an out-of-memory error, else LLSV_ERR_DEVICE.
+inline int hip_code(hipError_t e) { return e == hipErrorOutOfMemory ? LLSV_ERR_NOMEM : LLSV_ERR_DEVICE; }
+hipError_t dev_host_alloc(void** p, size_t bytes);
 
 // Thread-local error message for llsv_last_error().
 void set_error(const std::string& msg);

@@ -508,6 +508,11 @@ std::unordered_map<std::string, std::unique_ptr<JitEntry>>& cache() {
     return *c;
 }
 
+// The process keeps its own cache of compiled passes; keep the code-object manager from writing
+// one to disk.  Set once when the library is loaded, before it starts any thread (setenv must not
+// race with getenv on another thread, and the compiler worker reads the environment).
+__attribute__((constructor)) void jit_env_init() { setenv("AMD_COMGR_CACHE", "0", 0); }
+
 // Compiles the entries (state kBusy, set by the caller) as one hipRTC program, without the lock.
 void compile_batch(const std::vector<JitEntry*>& batch) {
     std::string src = kPrelude;
@@ -521,8 +526,6 @@ void compile_batch(const std::vector<JitEntry*>& batch) {
         body.replace(body.find("LLSV_KERNEL"), 11, names[i]);
         src += body;
     }
-    // The process keeps its own cache; keep the code-object manager from writing one to disk.
-    setenv("AMD_COMGR_CACHE", "0", 0);
     auto obj = std::make_shared<CodeObject>();
     std::string error;
     hiprtcProgram prog = nullptr;

@@ -28,6 +28,7 @@ hipError_t launch_reduce_prob(hipStream_t st, const v2d* psi, int n, int mode, i
                               double* out_dev);
 // <a|b> into out_dev[0..1].
 hipError_t launch_inner(hipStream_t st, const v2d* a, const v2d* b, int n, double* partials, double* out_dev);
-hipError_t launch_probabilities(hipStream_t st, const v2d* psi, int n, double* out_dev);
+// |psi_i|^2 for i < size into out_dev[i].
+hipError_t launch_probabilities(hipStream_t st, const v2d* psi, uint64_t size, double* out_dev);
 
 }  // namespace llsv

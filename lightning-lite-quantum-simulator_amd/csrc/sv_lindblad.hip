@@ -513,7 +513,7 @@ struct DevBuf {
         if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
-        const hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
+        const hipError_t e = dev_alloc(&p, bytes ? bytes : 16);
         if (e == hipSuccess) cap = bytes ? bytes : 16;
         return e;
     }
@@ -614,7 +614,7 @@ int llsv_expm(int dim, const double* a, double* out) {
 #define XK(expr)                                                                                      \
     do {                                                                                              \
         const hipError_t e_ = (expr);                                                                 \
-        if (e_ != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_expm: %s: %s", #expr, hipGetErrorString(e_)); \
+        if (e_ != hipSuccess) return fail(hip_code(e_), "llsv_expm: %s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
     XK(b.alloc(nn * 16));
     XK(term.alloc(nn * 16));
@@ -640,7 +640,7 @@ int llsv_zgemm(int M, int N, int K, const double* a, const double* b, double* c)
 #define GK(expr)                                                                                      \
     do {                                                                                              \
         const hipError_t e_ = (expr);                                                                 \
-        if (e_ != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_zgemm: %s: %s", #expr, hipGetErrorString(e_)); \
+        if (e_ != hipSuccess) return fail(hip_code(e_), "llsv_zgemm: %s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
     GK(da.alloc(size_t(M) * K * 16));
     GK(db.alloc(size_t(K) * N * 16));
@@ -705,7 +705,7 @@ int llsv_lplan_create(int dim, int kind, int64_t steps, double h, int n_nodes, c
 #define PK(expr)                                                                                      \
     do {                                                                                              \
         const hipError_t e_ = (expr);                                                                 \
-        if (e_ != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_lplan_create: %s: %s", #expr, hipGetErrorString(e_)); \
+        if (e_ != hipSuccess) return fail(hip_code(e_), "llsv_lplan_create: %s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
     PK(p->nodes.alloc(blk * size_t(n_nodes)));
     for (int k = 0; k < n_nodes; ++k) {
@@ -795,7 +795,7 @@ int llsv_lplan_create_h(int dim, int kind, int64_t steps, double h, int n_h, con
 #define PK(expr)                                                                                      \
     do {                                                                                              \
         const hipError_t e_ = (expr);                                                                 \
-        if (e_ != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_lplan_create_h: %s: %s", #expr, hipGetErrorString(e_)); \
+        if (e_ != hipSuccess) return fail(hip_code(e_), "llsv_lplan_create_h: %s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
     // channels: BT = [C_1^T | .. | C_n^T] (jumps) or [C^T .. | (C^+ C)^T ..] (diffusive), D x W
     const int W = (kind == 0 ? 1 : 2) * n * D;
@@ -894,7 +894,7 @@ int llsv_lplan_run(llsv_lplan* p, const double* psi0, int64_t n_traj, int64_t fi
 #define RK(expr)                                                                                      \
     do {                                                                                              \
         const hipError_t e_ = (expr);                                                                 \
-        if (e_ != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_lplan_run: %s: %s", #expr, hipGetErrorString(e_)); \
+        if (e_ != hipSuccess) return fail(hip_code(e_), "llsv_lplan_run: %s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
     // the reference's per-trajectory streams, drawn on the host by worker threads
     std::vector<double> table;

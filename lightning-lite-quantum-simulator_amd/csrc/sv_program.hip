@@ -468,8 +468,8 @@ int ensure_capacity(llsv_state* s, size_t bytes) {
     pc.dev = nullptr;
     pc.host = nullptr;
     pc.cap = 0;
-    if (hipMalloc(reinterpret_cast<void**>(&pc.dev), cap) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void**>(&pc.host), cap) != hipSuccess) {
+    if (dev_alloc(reinterpret_cast<void**>(&pc.dev), cap) != hipSuccess ||
+        dev_host_alloc(reinterpret_cast<void**>(&pc.host), cap) != hipSuccess) {
         (void)hipGetLastError();
         return fail(LLSV_ERR_NOMEM, "llsv_apply_program: cannot allocate %zu bytes of program buffers", cap);
     }

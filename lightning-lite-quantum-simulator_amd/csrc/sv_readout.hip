@@ -219,7 +219,7 @@ extern "C" int llsv_expect_pauli(llsv_state* s, const llsv_pauli* terms, int n_t
     const int grid = int(std::min<uint64_t>((size + 255) / 256, 1024));
     double* d = nullptr;  // [terms][kTermsPerPass] + partials[kTermsPerPass * grid] + out[kTermsPerPass]
     const size_t bytes = sizeof(PauliTerm) * kTermsPerPass + sizeof(double) * (size_t(kTermsPerPass) * grid + kTermsPerPass);
-    hipError_t e = n_terms ? hipMalloc(reinterpret_cast<void**>(&d), bytes) : hipSuccess;
+    hipError_t e = n_terms ? dev_alloc(reinterpret_cast<void**>(&d), bytes) : hipSuccess;
     PauliTerm* dterms = reinterpret_cast<PauliTerm*>(d);
     double* dpart = d ? d + 3 * kTermsPerPass : nullptr;
     double* dout = dpart ? dpart + size_t(kTermsPerPass) * grid : nullptr;
@@ -237,7 +237,7 @@ extern "C" int llsv_expect_pauli(llsv_state* s, const llsv_pauli* terms, int n_t
         if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     }
     if (d) (void)hipFree(d);
-    if (e != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_expect_pauli: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return fail(hip_code(e), "llsv_expect_pauli: %s", hipGetErrorString(e));
     double sum = 0.0;
     for (int t = 0; t < n_terms; ++t) sum += host[t];
     if (per_term) std::memcpy(per_term, host.data(), sizeof(double) * n_terms);
@@ -254,7 +254,7 @@ extern "C" int llsv_sample(llsv_state* s, uint64_t shots, uint64_t seed, uint64_
     const uint64_t nchunks = size / chunk;
     const size_t bytes = sizeof(double) * (2 * nchunks + 8) + sizeof(uint64_t) * shots;
     unsigned char* d = nullptr;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&d), bytes);
+    hipError_t e = dev_alloc(reinterpret_cast<void**>(&d), bytes);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return fail(LLSV_ERR_NOMEM, "llsv_sample: cannot allocate %zu bytes", bytes);

@@ -119,8 +119,7 @@ __global__ __launch_bounds__(256) void k_probabilities(const v2d* __restrict__ p
     }
 }
 
-hipError_t launch_probabilities(hipStream_t st, const v2d* psi, int n, double* out_dev) {
-    const uint64_t size = uint64_t(1) << n;
+hipError_t launch_probabilities(hipStream_t st, const v2d* psi, uint64_t size, double* out_dev) {
     hipLaunchKernelGGL(k_probabilities, dim3(grid_for((size + 255) / 256)), dim3(256), 0, st, psi, size, out_dev);
     return hipGetLastError();
 }

@@ -21,4 +21,11 @@ struct llsv_state {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t marks[LLSV_MARK_SLOTS] = {};
     llsv::ProgramCache* program = nullptr;
+    // llsv_probabilities: device chunk of kProbChunk doubles, allocated on first use and kept
+    double* prob_chunk = nullptr;
+    // llsv_apply_dense: pinned upload slots (matrix + offset table), each reused only after the
+    // copy that last read it has completed (its event)
+    unsigned char* stage = nullptr;
+    hipEvent_t stage_ev[8] = {};
+    int stage_next = 0;
 };

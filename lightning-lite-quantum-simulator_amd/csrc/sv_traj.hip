@@ -1,14 +1,19 @@
 // sv_traj.hip -- batched gate-level noisy trajectories in one persistent launch.
 //
-// Workgroups (256 threads) take items from a device work counter until it runs past the
-// batch, so a mixed batch of 10..16-qubit circuits balances itself (items are handed out in
-// decreasing size).  Each workgroup owns a state slot of 2^max_n amplitudes in the workspace
-// (256 KiB at 14 qubits, resident in L2 / Infinity Cache while the item runs).  Per gate:
-//   pass 1: apply the gate and accumulate the joint probabilities of its qubits;
-//   thread 0: draw Philox4x32-10 uniforms (key = seed, counter = (trajectory, gate, site)),
-//             decide AD then PD on each touched qubit from the joint weights, updated in closed
-//             form after each outcome, and form the product operator (normalised);
-//   pass 2: apply that operator.
+// Workgroups take items from a device work counter until it runs past the batch, so a mixed
+// batch of 10..16-qubit circuits balances itself (items are handed out in decreasing size).  Each
+// workgroup owns a state slot of 2^max_n amplitudes in the workspace (256 KiB at 14 qubits,
+// resident in L2 / Infinity Cache while the item runs).
+//
+// Every noise site's Philox4x32-10 uniform (key = seed, counter = (trajectory, gate, site)) is
+// drawn before its gate runs.  A site whose uniform lies below 1 - rate cannot jump, whatever the
+// state; there the operator is the fixed no-jump product, which joins the pending 2x2 operator of
+// its qubit instead of touching the state.  Pending operators reach the state inside the pass of
+// the next CNOT on their qubit, or when a site that may jump needs the joint weights of its
+// qubits: then every pending operator is applied (two qubits per pass), the weights are reduced
+// over the workgroup, and each thread takes the same decisions from them (AD then PD on each
+// touched qubit, weights updated in closed form after each outcome).  The state is renormalised
+// only at those decisions; <Z_q> divides by the final norm.
 // Every wave reaches the exit test of the work loop; nothing waits on another workgroup.
 #include <hip/hip_runtime.h>
 
@@ -155,94 +160,116 @@ __device__ inline void block_sums(double* v, double (*lds)[WG / 64], double* out
 
 __global__ void k_traj_reset(unsigned* counter) { *counter = 0u; }
 
-// One-qubit operator m on qubit q; with PROBS, v[0] / v[1] accumulate the weight of q = 0 / 1.
-template <int WG, bool PROBS>
-__device__ inline void pass_1q(v2d* s, uint64_t N, int q, const double* m, double* v) {
-    const uint64_t lomask = (uint64_t(1) << q) - 1, st = uint64_t(1) << q;
+// Joint weights of bits (sa, sb) of amplitude i: v[2 * bit_sa + bit_sb] += p (selects, so v stays
+// in registers).  A one-qubit statistic uses sa = sb: v[0] / v[3] get the weight of bit 0 / 1.
+__device__ __forceinline__ void acc4(double* v, uint64_t i, int sa, int sb, double p) {
+    const int k = int(((i >> sa) & 1) * 2 + ((i >> sb) & 1));
+    v[0] += k == 0 ? p : 0.0;
+    v[1] += k == 1 ? p : 0.0;
+    v[2] += k == 2 ? p : 0.0;
+    v[3] += k == 3 ? p : 0.0;
+}
+
+// Operator ma on qubit a, one pass over the pairs.
+template <int WG, bool STATS>
+__device__ inline void pass_one(v2d* s, uint64_t N, int a, const double* ma, int sa, int sb, double* v) {
+    const uint64_t lomask = (uint64_t(1) << a) - 1, ab = uint64_t(1) << a;
     for (uint64_t p = threadIdx.x; p < N / 2; p += WG) {
-        const uint64_t i0 = ((p & ~lomask) << 1) | (p & lomask);
-        const v2d a0 = s[i0], a1 = s[i0 + st];
-        const v2d b0 = ap(m, a0, a1), b1 = ap(m + 4, a0, a1);
-        s[i0] = b0;
-        s[i0 + st] = b1;
-        if (PROBS) {
-            v[0] += nrm2(b0);
-            v[1] += nrm2(b1);
+        const uint64_t i = ((p & ~lomask) << 1) | (p & lomask);
+        const v2d x0 = s[i], x1 = s[i | ab];
+        const v2d y0 = ap(ma, x0, x1), y1 = ap(ma + 4, x0, x1);
+        s[i] = y0;
+        s[i | ab] = y1;
+        if (STATS) {
+            acc4(v, i, sa, sb, nrm2(y0));
+            acc4(v, i | ab, sa, sb, nrm2(y1));
         }
     }
 }
 
-// The pending noise operator k on qubit a, then gate m on qubit b != a, in one pass over
-// groups of 4 amplitudes; v[0] / v[1] accumulate the weight of b = 0 / 1 after both.
-template <int WG>
-__device__ inline void pass_k_then_1q(v2d* s, uint64_t N, int a, const double* k, int b, const double* m, double* v) {
+// ma on qubit a and mb on qubit b (a != b; they commute), then, with CNOT, the CNOT with control
+// a and target b; one pass over groups of 4 amplitudes.
+template <int WG, bool STATS, bool CNOT>
+__device__ inline void pass_two(v2d* s, uint64_t N, int a, const double* ma, int b, const double* mb, int sa, int sb,
+                                double* v) {
     const int lo = a < b ? a : b, hi = a < b ? b : a;
     const uint64_t ab = uint64_t(1) << a, bb = uint64_t(1) << b;
     for (uint64_t p = threadIdx.x; p < N / 4; p += WG) {
         const uint64_t i = insert0(insert0(p, lo), hi);
         const v2d x00 = s[i], x10 = s[i | ab], x01 = s[i | bb], x11 = s[i | ab | bb];  // x(a bit)(b bit)
-        const v2d y00 = ap(k, x00, x10), y10 = ap(k + 4, x00, x10);
-        const v2d y01 = ap(k, x01, x11), y11 = ap(k + 4, x01, x11);
-        const v2d z00 = ap(m, y00, y01), z01 = ap(m + 4, y00, y01);
-        const v2d z10 = ap(m, y10, y11), z11 = ap(m + 4, y10, y11);
+        const v2d y00 = ap(ma, x00, x10), y10 = ap(ma + 4, x00, x10);
+        const v2d y01 = ap(ma, x01, x11), y11 = ap(ma + 4, x01, x11);
+        const v2d z00 = ap(mb, y00, y01), z01 = ap(mb + 4, y00, y01);
+        const v2d z10 = ap(mb, y10, y11), z11 = ap(mb + 4, y10, y11);
+        const v2d f10 = CNOT ? z11 : z10, f11 = CNOT ? z10 : z11;  // control a set: target b flipped
         s[i] = z00;
-        s[i | ab] = z10;
+        s[i | ab] = f10;
         s[i | bb] = z01;
-        s[i | ab | bb] = z11;
-        v[0] += nrm2(z00) + nrm2(z10);
-        v[1] += nrm2(z01) + nrm2(z11);
+        s[i | ab | bb] = f11;
+        if (STATS) {
+            acc4(v, i, sa, sb, nrm2(z00));
+            acc4(v, i | ab, sa, sb, nrm2(f10));
+            acc4(v, i | bb, sa, sb, nrm2(z01));
+            acc4(v, i | ab | bb, sa, sb, nrm2(f11));
+        }
     }
 }
 
-// CNOT(c, t); v[] accumulates the joint weights (c, t) = 00, 01, 10, 11 after the gate.
-template <int WG>
-__device__ inline void pass_cnot(v2d* s, uint64_t N, int c, int t, double* v) {
-    const int lo = c < t ? c : t, hi = c < t ? t : c;
-    const uint64_t cb = uint64_t(1) << c, tb = uint64_t(1) << t;
-    for (uint64_t p = threadIdx.x; p < N / 4; p += WG) {
-        const uint64_t b = insert0(insert0(p, lo), hi);
-        const v2d a00 = s[b], a01 = s[b | tb], a10 = s[b | cb], a11 = s[b | cb | tb];
-        s[b | cb] = a11;  // control set: swap target
-        s[b | cb | tb] = a10;
-        v[0] += nrm2(a00);
-        v[1] += nrm2(a01);
-        v[2] += nrm2(a11);
-        v[3] += nrm2(a10);
+// Bits of the possible-jump mask of a gate: site s may jump only if u_s >= 1 - rate (a jump needs
+// u >= 1 - rate * p with p <= 1, both sides rounded the way decide() rounds them).
+__device__ inline unsigned possible_sites(const TG& g, const double* gamma, const double* lam, int first_rate,
+                                          uint64_t seed, uint64_t traj, uint32_t gi) {
+    unsigned m = 0;
+    const int nq = g.kind == 0 ? 1 : 2;
+    for (int k = 0; k < nq; ++k) {
+        const int q = k == 0 ? g.q0 : g.q1;
+        const double r0 = gamma[first_rate + q], r1 = lam[first_rate + q];
+        if (uniform(seed, traj, gi, uint32_t(2 * k)) >= __dsub_rn(1.0, r0)) m |= 1u << (2 * k);
+        if (uniform(seed, traj, gi, uint32_t(2 * k + 1)) >= __dsub_rn(1.0, r1)) m |= 1u << (2 * k + 1);
     }
+    return m;
 }
 
-// Noise operators kc on qubit c, then kt on qubit t (they act on different qubits).
-template <int WG>
-__device__ inline void pass_k2(v2d* s, uint64_t N, int c, int t, const double* kc, const double* kt) {
-    const int lo = c < t ? c : t, hi = c < t ? t : c;
-    const uint64_t cb = uint64_t(1) << c, tb = uint64_t(1) << t;
-    for (uint64_t p = threadIdx.x; p < N / 4; p += WG) {
-        const uint64_t b = insert0(insert0(p, lo), hi);
-        v2d a00 = s[b], a01 = s[b | tb], a10 = s[b | cb], a11 = s[b | cb | tb];
-        const v2d c00 = ap(kc, a00, a10), c10 = ap(kc + 4, a00, a10);
-        const v2d c01 = ap(kc, a01, a11), c11 = ap(kc + 4, a01, a11);
-        s[b] = ap(kt, c00, c01);
-        s[b | tb] = ap(kt + 4, c00, c01);
-        s[b | cb] = ap(kt, c10, c11);
-        s[b | cb | tb] = ap(kt + 4, c10, c11);
-    }
+// The no-jump product PD0 AD0 = diag(1, sqrt(1 - l) sqrt(1 - g)) (unnormalised).
+__device__ inline void no_jump_op(double g, double l, double* k) {
+    for (int j = 0; j < 8; ++j) k[j] = 0.0;
+    k[0] = 1.0;
+    k[6] = sqrt(1.0 - l) * sqrt(1.0 - g);
 }
 
-// WG threads per trajectory.  FUSE: the noise operator drawn after a gate is not applied in a
-// pass of its own but inside the next gate's pass when that gate is a one-qubit gate (multiplied
-// into it when both act on the same qubit); before a CNOT, and after the last gate, it gets its
-// own pass.  The operators and the order they act in are the same either way.
-template <int WG, bool FUSE>
+__device__ inline void copy8(const double* a, double* b) {
+    for (int j = 0; j < 8; ++j) b[j] = a[j];
+}
+
+constexpr int TRAJ_CHUNK = 256;  // gates whose possible-jump masks are drawn together
+
+// WG threads per trajectory.  Decisions are the reference's (src/TrajectorySolver.h:425, with the
+// joint-weight updates of decide()); what changes is when the state is touched:
+//   * every site's uniform is drawn ahead (TRAJ_CHUNK gates at a time, one gate per thread); a
+//     site with u < 1 - rate cannot jump whatever the state, so its operator is the fixed no-jump
+//     product and no probability is needed;
+//   * the state is not renormalised between decisions (a decision only uses ratios of weights),
+//     so a gate with no possible jump is a 2x2 product on the host-free side: thread 0 multiplies
+//     gate and no-jump operator into the pending operator of the gate's qubit (s_pend);
+//   * DEFER: pending operators stay pending until a CNOT touches their qubit (applied in the
+//     CNOT's pass) or a possible jump needs weights (then every pending operator is applied, two
+//     qubits per pass, the last pass accumulating the weights); !DEFER: each gate's operators are
+//     applied right after it, one pass per gate (plus one for a CNOT's noise).
+// At a possible jump every thread runs decide() on the same block sums, so the outcome needs no
+// broadcast; the chosen operator, scaled to renormalise, becomes the qubit's pending operator.
+// <Z_q> divides by the final norm.
+template <int WG, bool DEFER>
 __global__ __launch_bounds__(WG) void k_traj(const TItem* __restrict__ items, int n_items,
                                              const int* __restrict__ order, const TG* __restrict__ gates,
                                              const double* __restrict__ gamma, const double* __restrict__ lam,
                                              const uint64_t* __restrict__ seedp, v2d* __restrict__ ws,
                                              uint64_t slot_size, unsigned* __restrict__ counter,
                                              double* __restrict__ zout, unsigned* __restrict__ jout,
-                                             uint64_t* __restrict__ hout) {
+                                             uint64_t* __restrict__ hout, int norm_state) {
     __shared__ int s_item;
-    __shared__ double s_k[2][8];
-    __shared__ double lds[20][WG / 64];
+    __shared__ double s_pend[20][8];
+    __shared__ unsigned char s_pm[TRAJ_CHUNK];
+    __shared__ double lds[8][WG / 64];
     v2d* s = ws + uint64_t(blockIdx.x) * slot_size;
     const int tid = threadIdx.x;
     const uint64_t seed = *seedp;
@@ -264,97 +291,152 @@ __global__ __launch_bounds__(WG) void k_traj(const TItem* __restrict__ items, in
             z.y = 0.0;
             s[i] = z;
         }
-        __syncthreads();
         unsigned jumps = 0;
         uint64_t hash = 0xCBF29CE484222325ull;
-        int pend = 0, pq0 = 0, pq1 = 0;  // pending noise operator: none, on pq0 (s_k[0]), or on pq0 and pq1
-        auto flush = [&]() {
-            double k0[8], k1[8];
-            for (int j = 0; j < 8; ++j) {
-                k0[j] = s_k[0][j];
-                k1[j] = s_k[1][j];
-            }
-            if (pend == 1)
-                pass_1q<WG, false>(s, N, pq0, k0, nullptr);
-            else
-                pass_k2<WG>(s, N, pq0, pq1, k0, k1);
-            pend = 0;
+        uint32_t pmask = 0;  // qubits with a pending operator in s_pend (the same in every thread)
+
+        // Applies the pending operators of the qubits in `mask`, two per pass; the last pass
+        // accumulates the joint weights of bits (sa, sb) into v when STATS.
+        auto flush = [&](uint32_t mask, bool stats, int sa, int sb, double* v) {
             __syncthreads();
+            while (mask) {
+                const int a = __builtin_ctz(mask);
+                mask &= mask - 1;
+                double ma[8], mb[8];
+                copy8(s_pend[a], ma);
+                if (mask) {
+                    const int b = __builtin_ctz(mask);
+                    mask &= mask - 1;
+                    copy8(s_pend[b], mb);
+                    if (stats && !mask)
+                        pass_two<WG, true, false>(s, N, a, ma, b, mb, sa, sb, v);
+                    else
+                        pass_two<WG, false, false>(s, N, a, ma, b, mb, sa, sb, v);
+                } else if (stats) {
+                    pass_one<WG, true>(s, N, a, ma, sa, sb, v);
+                } else {
+                    pass_one<WG, false>(s, N, a, ma, sa, sb, v);
+                }
+                __syncthreads();
+            }
         };
+        // s_pend[q] <- a * s_pend[q] (or a when q has nothing pending); thread 0 only.
+        auto push = [&](int q, const double* a) {
+            if (tid == 0) {
+                if (pmask & (1u << q))
+                    mm2(a, s_pend[q], s_pend[q]);
+                else
+                    copy8(a, s_pend[q]);
+            }
+            pmask |= 1u << q;
+        };
+
         for (int gi = 0; gi < item.n_gates; ++gi) {
+            if (gi % TRAJ_CHUNK == 0) {
+                __syncthreads();
+                for (int k = tid; k < TRAJ_CHUNK && gi + k < item.n_gates; k += WG)
+                    s_pm[k] = (unsigned char)possible_sites(gates[item.first_gate + gi + k], gamma, lam, item.first_rate,
+                                                            seed, item.traj, uint32_t(gi + k));
+                __syncthreads();
+            }
+            const unsigned pm = s_pm[gi % TRAJ_CHUNK];
             const TG g = gates[item.first_gate + gi];
             if (g.kind == 0) {
                 const int q = g.q0;
-                double v[2] = {0.0, 0.0};
-                if (FUSE && pend == 1) {
-                    double k[8];
-                    for (int j = 0; j < 8; ++j) k[j] = s_k[0][j];
-                    if (pq0 == q) {
-                        double mk[8];
-                        mm2(g.m, k, mk);
-                        pass_1q<WG, true>(s, N, q, mk, v);
-                    } else {
-                        pass_k_then_1q<WG>(s, N, pq0, k, q, g.m, v);
-                    }
-                    pend = 0;
+                const double gq = gamma[item.first_rate + q], lq = lam[item.first_rate + q];
+                if (!pm) {
+                    double k[8], kg[8];
+                    no_jump_op(gq, lq, k);
+                    mm2(k, g.m, kg);
+                    push(q, kg);
                 } else {
-                    if (pend) flush();
-                    pass_1q<WG, true>(s, N, q, g.m, v);
-                }
-                double P2[2];
-                block_sums<WG, 2>(v, lds, P2);
-                if (tid == 0) {
-                    double P[2][2] = {{P2[0], 0.0}, {P2[1], 0.0}};
+                    push(q, g.m);
+                    double v[4] = {0.0, 0.0, 0.0, 0.0}, P4[4];
+                    flush(pmask, true, q, q, v);
+                    pmask = 0;
+                    block_sums<WG, 4>(v, lds, P4);
+                    double P[2][2] = {{P4[0], 0.0}, {P4[3], 0.0}};
                     double k[8];
-                    jumps += decide(P, gamma[item.first_rate + q], lam[item.first_rate + q], seed, item.traj,
-                                    uint32_t(gi), 0, k, &hash);
+                    jumps += decide(P, gq, lq, seed, item.traj, uint32_t(gi), 0, k, &hash);
                     const double nr = 1.0 / sqrt(P[0][0] + P[1][0]);
-                    for (int j = 0; j < 8; ++j) s_k[0][j] = k[j] * nr;
+                    for (int j = 0; j < 8; ++j) k[j] *= nr;
+                    push(q, k);
                 }
-                __syncthreads();
-                pend = 1;
-                pq0 = q;
-                if (!FUSE) flush();
+                if (!DEFER) {
+                    flush(pmask, false, 0, 0, nullptr);
+                    pmask = 0;
+                }
             } else {
-                if (pend) flush();
                 const int c = g.q0, t = g.q1;
-                double v[4] = {0.0, 0.0, 0.0, 0.0};
-                pass_cnot<WG>(s, N, c, t, v);
-                double P4[4];
-                block_sums<WG, 4>(v, lds, P4);
-                if (tid == 0) {
-                    double P[2][2] = {{P4[0], P4[1]}, {P4[2], P4[3]}};
-                    double kc[8], kt[8];
-                    jumps += decide(P, gamma[item.first_rate + c], lam[item.first_rate + c], seed, item.traj,
-                                    uint32_t(gi), 0, kc, &hash);
-                    double Q[2][2] = {{P[0][0], P[1][0]}, {P[0][1], P[1][1]}};
-                    jumps += decide(Q, gamma[item.first_rate + t], lam[item.first_rate + t], seed, item.traj,
-                                    uint32_t(gi), 2, kt, &hash);
-                    const double nr = 1.0 / sqrt((Q[0][0] + Q[0][1]) + (Q[1][0] + Q[1][1]));
-                    for (int j = 0; j < 8; ++j) {
-                        s_k[0][j] = kc[j];
-                        s_k[1][j] = kt[j] * nr;
-                    }
+                const uint32_t ct = (1u << c) | (1u << t);
+                if (pm) {  // weights after the CNOT need every other pending operator applied first
+                    flush(pmask & ~ct, false, 0, 0, nullptr);
+                    pmask &= ct;
                 }
                 __syncthreads();
-                pend = 2;
-                pq0 = c;
-                pq1 = t;
-                if (!FUSE) flush();
+                double mc[8] = {1, 0, 0, 0, 0, 0, 1, 0}, mt[8] = {1, 0, 0, 0, 0, 0, 1, 0};
+                if (pmask & (1u << c)) copy8(s_pend[c], mc);
+                if (pmask & (1u << t)) copy8(s_pend[t], mt);
+                double v[4] = {0.0, 0.0, 0.0, 0.0};
+                if (pm)
+                    pass_two<WG, true, true>(s, N, c, mc, t, mt, c, t, v);
+                else
+                    pass_two<WG, false, true>(s, N, c, mc, t, mt, c, t, v);
+                __syncthreads();
+                pmask &= ~ct;
+                const double gc = gamma[item.first_rate + c], lc = lam[item.first_rate + c];
+                const double gt = gamma[item.first_rate + t], lt = lam[item.first_rate + t];
+                double kc[8], kt[8];
+                if (!pm) {
+                    no_jump_op(gc, lc, kc);
+                    no_jump_op(gt, lt, kt);
+                } else {
+                    double P4[4];
+                    block_sums<WG, 4>(v, lds, P4);
+                    double P[2][2] = {{P4[0], P4[1]}, {P4[2], P4[3]}};
+                    jumps += decide(P, gc, lc, seed, item.traj, uint32_t(gi), 0, kc, &hash);
+                    double Q[2][2] = {{P[0][0], P[1][0]}, {P[0][1], P[1][1]}};
+                    jumps += decide(Q, gt, lt, seed, item.traj, uint32_t(gi), 2, kt, &hash);
+                    const double nr = 1.0 / sqrt((Q[0][0] + Q[0][1]) + (Q[1][0] + Q[1][1]));
+                    for (int j = 0; j < 8; ++j) kt[j] *= nr;
+                }
+                push(c, kc);
+                push(t, kt);
+                if (!DEFER) {
+                    flush(pmask, false, 0, 0, nullptr);
+                    pmask = 0;
+                }
             }
         }
-        if (pend) flush();
-        double zq[20];
-        for (int q = 0; q < 20; ++q) zq[q] = 0.0;
-        for (uint64_t i = tid; i < N; i += WG) {
-            const double p = nrm2(s[i]);
-            for (int q = 0; q < 20; ++q)
-                if (q < n) zq[q] += ((i >> q) & 1) ? -p : p;
+        flush(pmask, false, 0, 0, nullptr);
+        pmask = 0;
+        // <Z_q> = sum_i p_i (1 - 2 bit_q(i)) / sum_i p_i, seven qubits per read of the state (few
+        // accumulators, so the loop stays in registers); the norm comes with the first seven.
+        double norm = 1.0;
+        for (int q0 = 0; q0 < n; q0 += 7) {
+            double zq[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) zq[j] = 0.0;
+            for (uint64_t i = tid; i < N; i += WG) {
+                const double p = nrm2(s[i]);
+                zq[7] += p;
+#pragma unroll
+                for (int j = 0; j < 7; ++j) zq[j] += ((i >> (q0 + j)) & 1) ? -p : p;
+            }
+            double Z[8];
+            block_sums<WG, 8>(zq, lds, Z);
+            if (q0 == 0) norm = Z[7];
+            if (tid == 0) {
+#pragma unroll
+                for (int j = 0; j < 7; ++j)
+                    if (q0 + j < n) zout[size_t(it) * LLSV_TRAJ_ZSTRIDE + q0 + j] = Z[j] / norm;
+            }
         }
-        double Z[20];
-        block_sums<WG, 20>(zq, lds, Z);
+        if (norm_state) {  // the caller reads this state back: normalise it
+            const double r = 1.0 / sqrt(norm);
+            for (uint64_t i = tid; i < N; i += WG) s[i] = s[i] * r;
+        }
         if (tid == 0) {
-            for (int q = 0; q < n; ++q) zout[size_t(it) * LLSV_TRAJ_ZSTRIDE + q] = Z[q];
             jout[it] = jumps;
             hout[it] = hash;
         }
@@ -384,11 +466,12 @@ int traj_blocks(const TrajConfig& c, int n_items, int maxn, int cus) {
 
 hipError_t launch_traj(const TrajConfig& c, int blocks, hipStream_t st, const TItem* items, int n_items,
                        const int* order, const TG* gates, const double* gamma, const double* lam, const uint64_t* seed,
-                       v2d* ws, uint64_t slot, unsigned* counter, double* z, unsigned* j, uint64_t* h) {
+                       v2d* ws, uint64_t slot, unsigned* counter, double* z, unsigned* j, uint64_t* h,
+                       int norm_state = 0) {
     hipLaunchKernelGGL(k_traj_reset, dim3(1), dim3(1), 0, st, counter);
 #define LLSV_TRAJ_LAUNCH(WG, F)                                                                                   \
     hipLaunchKernelGGL((k_traj<WG, F>), dim3(blocks), dim3(WG), 0, st, items, n_items, order, gates, gamma, lam, \
-                       seed, ws, slot, counter, z, j, h)
+                       seed, ws, slot, counter, z, j, h, norm_state)
     if (c.wg == 1024) {
         if (c.fuse) LLSV_TRAJ_LAUNCH(1024, true); else LLSV_TRAJ_LAUNCH(1024, false);
     } else if (c.wg == 512) {
@@ -485,7 +568,7 @@ int traj_upload(TrajDev* d, const llsv_traj_item* items, int n_items, const std:
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
     const size_t total = al(b_items) + 2 * al(b_order) + al(b_gates) + 2 * al(b_rates) + al(b_z) + al(b_j) + al(b_h) +
                          512 + al(b_ws);
-    if (hipMalloc(reinterpret_cast<void**>(&d->buf), total) != hipSuccess) {
+    if (dev_alloc(reinterpret_cast<void**>(&d->buf), total) != hipSuccess) {
         (void)hipGetLastError();
         d->buf = nullptr;
         return fail(LLSV_ERR_NOMEM, "%s: cannot allocate %zu bytes", who, total);
@@ -633,7 +716,7 @@ extern "C" int llsv_traj_run(const llsv_traj_item* items, int n_items, const lls
                 if (e == hipSuccess) e = hipMemcpyAsync(d.sample, &zero, sizeof(int), hipMemcpyHostToDevice, st);
                 if (e == hipSuccess)
                     e = launch_traj(cfg, 1, st, d.items, 1, d.sample, d.gates, d.gamma, d.lam, d.seed, d.ws, d.slot,
-                                    d.counter, d.z, d.j, d.h);
+                                    d.counter, d.z, d.j, d.h, 1);
                 if (e == hipSuccess)
                     e = hipMemcpyAsync(state, d.ws, sizeof(v2d) << items[0].n_qubits, hipMemcpyDeviceToHost, st);
             }
@@ -683,10 +766,10 @@ extern "C" int llsv_traj_plan_create(const llsv_traj_item* items, int n_items, c
         tune_cache()[shape] = p->cfg;
     }
     p->blocks = std::min(traj_blocks(p->cfg, n_items, shape.maxn, cus), max_blocks);
-    if (hipHostMalloc(reinterpret_cast<void**>(&p->h_seed), 256) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void**>(&p->h_z), sizeof(double) * LLSV_TRAJ_ZSTRIDE * n_items) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void**>(&p->h_j), sizeof(unsigned) * n_items) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void**>(&p->h_h), sizeof(uint64_t) * n_items) != hipSuccess) {
+    if (dev_host_alloc(reinterpret_cast<void**>(&p->h_seed), 256) != hipSuccess ||
+        dev_host_alloc(reinterpret_cast<void**>(&p->h_z), sizeof(double) * LLSV_TRAJ_ZSTRIDE * n_items) != hipSuccess ||
+        dev_host_alloc(reinterpret_cast<void**>(&p->h_j), sizeof(unsigned) * n_items) != hipSuccess ||
+        dev_host_alloc(reinterpret_cast<void**>(&p->h_h), sizeof(uint64_t) * n_items) != hipSuccess) {
         (void)hipGetLastError();
         return fail(LLSV_ERR_NOMEM, "llsv_traj_plan_create: cannot allocate pinned result buffers");
     }

@@ -434,7 +434,7 @@ struct ZBuf {
         if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
-        const hipError_t e = hipMalloc(&p, bytes);
+        const hipError_t e = dev_alloc(&p, bytes);
         if (e == hipSuccess) cap = bytes;
         return e;
     }
@@ -449,7 +449,7 @@ struct PinBuf {
         if (p) (void)hipHostFree(p);
         p = nullptr;
         cap = 0;
-        const hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+        const hipError_t e = dev_host_alloc(&p, bytes);
         if (e == hipSuccess) cap = bytes;
         return e;
     }
@@ -560,7 +560,7 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
 #define ZK(expr)                                                                                         \
     do {                                                                                                 \
         const hipError_t e_ = (expr);                                                                    \
-        if (e_ != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_noisy_run: %s: %s", #expr, hipGetErrorString(e_)); \
+        if (e_ != hipSuccess) return fail(hip_code(e_), "llsv_noisy_run: %s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
     ZK(b_apps.need(sizeof(ZApp) * apps.size()));
     ZK(hipMemcpy(b_apps.p, apps.data(), sizeof(ZApp) * apps.size(), hipMemcpyHostToDevice));

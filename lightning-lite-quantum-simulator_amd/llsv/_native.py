@@ -81,6 +81,7 @@ SIGNATURES = {
     "llsv_stream": (_P, [_P]),
     "llsv_timer_start": (_I, [_P]),
     "llsv_timer_stop": (_I, [_P, ctypes.POINTER(ctypes.c_float)]),
+    "llsv_test_fail_allocation": (ctypes.c_long, [ctypes.c_long]),
     "llsv_last_error": (ctypes.c_char_p, []),
     "llsv_version": (ctypes.c_char_p, []),
 }
