@@ -243,10 +243,26 @@ int llsv_expect_pauli(llsv_state* s, const llsv_pauli* terms, int n_terms, doubl
 /* Seeded measurement sampling in the computational basis: out[k] (host, shots entries) is the
  * smallest basis index i with CDF(i) > u_k * CDF(last), CDF(i) = sum_{j <= i} |psi_j|^2 and
  * u_k = (x >> 11) 2^-53, x the first 64 bits of Philox4x32-10(key = seed, counter = (k low,
- * k high, 0x53414D50, 0)).  Indices with zero probability are never returned.  The state is
+ * k high, 0x53414D50, 0)).  Indices with zero probability are never returned; a state whose total
+ * probability is zero or not finite is LLSV_ERR_NUMERIC (out is then unspecified).  The state is
  * unchanged.  No counterpart in the reference (its scripts sample from probabilities() in
  * NumPy). */
 int llsv_sample(llsv_state* s, uint64_t shots, uint64_t seed, uint64_t* out);
+
+/* Device scratch held by llsv_sample_noalloc on each GPU it runs on, until the process ends. */
+#define LLSV_SAMPLE_SCRATCH_BYTES (32ull << 20)
+
+/* llsv_sample without any allocation: the same draws and the same indices, with the chunk sums,
+ * prefixes and sampled indices kept in a fixed device array of LLSV_SAMPLE_SCRATCH_BYTES per GPU
+ * instead of a buffer allocated per call.  Needs 16 * max(1, 2^n / 4096) + 64 + 8 * shots <=
+ * LLSV_SAMPLE_SCRATCH_BYTES (so n <= 32; at 30 qubits up to 3.6 M shots per call), else
+ * LLSV_ERR_ARG.  Calls no hipMalloc, hipFree or other allocating routine.  All calls on one GPU
+ * share its scratch, which keeps the last call's contents: calls that overlap on one GPU (two
+ * threads, or states with different streams) race and must be serialised by the caller.  A
+ * zero or non-finite total probability is LLSV_ERR_NUMERIC (as for llsv_sample).  No counterpart
+ * in the reference; for callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation. */
+int llsv_sample_noalloc(llsv_state* s, uint64_t shots, uint64_t seed, uint64_t* out);
 
 /* ---- batched noisy trajectories ---------------------------------------------- */
 

@@ -16,7 +16,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/llsv.h"
@@ -245,20 +247,25 @@ extern "C" int llsv_expect_pauli(llsv_state* s, const llsv_pauli* terms, int n_t
     return LLSV_OK;
 }
 
-extern "C" int llsv_sample(llsv_state* s, uint64_t shots, uint64_t seed, uint64_t* out) {
-    if (!s) return fail(LLSV_ERR_HANDLE, "llsv_sample: null state handle");
-    if (shots && !out) return fail(LLSV_ERR_ARG, "llsv_sample: out is null");
-    if (shots == 0) return LLSV_OK;
+// Scratch of llsv_sample_noalloc: chunk sums, prefixes, the total and the sampled indices, one
+// copy per device for the life of the process.  Host code never passes `scratch` itself to a
+// kernel (on the host it is a stand-in, not a device address): it uses the address that
+// hipGetSymbolAddress returns for the device (scratch_on below).
+static __device__ __align__(256) unsigned char scratch[LLSV_SAMPLE_SCRATCH_BYTES];
+
+namespace {
+
+// Device bytes a sampling call needs: chunk sums and prefixes, the total, the sampled indices.
+uint64_t sample_bytes(uint64_t size, uint64_t shots) {
+    const uint64_t chunk = std::min<uint64_t>(size, uint64_t(1) << kChunkBits);
+    return sizeof(double) * (2 * (size / chunk) + 8) + sizeof(uint64_t) * shots;
+}
+
+// The sampling of llsv_sample with its device buffer d (sample_bytes(size, shots) bytes) given.
+int sample_into(llsv_state* s, uint64_t shots, uint64_t seed, uint64_t* out, unsigned char* d, const char* who) {
     const uint64_t size = s->size;
     const uint64_t chunk = std::min<uint64_t>(size, uint64_t(1) << kChunkBits);
     const uint64_t nchunks = size / chunk;
-    const size_t bytes = sizeof(double) * (2 * nchunks + 8) + sizeof(uint64_t) * shots;
-    unsigned char* d = nullptr;
-    hipError_t e = dev_alloc(reinterpret_cast<void**>(&d), bytes);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(LLSV_ERR_NOMEM, "llsv_sample: cannot allocate %zu bytes", bytes);
-    }
     double* sums = reinterpret_cast<double*>(d);
     double* prefix = sums + nchunks;
     double* total = prefix + nchunks;
@@ -268,10 +275,70 @@ extern "C" int llsv_sample(llsv_state* s, uint64_t shots, uint64_t seed, uint64_
     hipLaunchKernelGGL(k_chunk_prefix, dim3(1), dim3(1024), 0, s->stream, sums, nchunks, prefix, total);
     hipLaunchKernelGGL(k_sample, dim3(unsigned(std::min<uint64_t>((shots + 3) / 4, 1u << 16))), dim3(256), 0, s->stream,
                        s->amps, size, chunk, nchunks, prefix, sums, total, seed, shots, dout);
-    e = hipGetLastError();
+    hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(uint64_t) * shots, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->result_host, total, sizeof(double), hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    (void)hipFree(d);
-    if (e != hipSuccess) return fail(LLSV_ERR_DEVICE, "llsv_sample: %s", hipGetErrorString(e));
+    if (e != hipSuccess) return fail(LLSV_ERR_DEVICE, "%s: %s", who, hipGetErrorString(e));
+    const double tot = s->result_host[0];
+    if (!(tot > 0.0) || !std::isfinite(tot))
+        return fail(LLSV_ERR_NUMERIC, "%s: the state has no probability to sample from (norm %g)", who, tot);
     return LLSV_OK;
+}
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, looked up once per device with that device current.
+int scratch_on(int device, unsigned char** out, const char* who) {
+    if (device < 0 || device >= kMaxDevices) return fail(LLSV_ERR_DEVICE, "%s: device %d out of range", who, device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e == hipSuccess && cur != device) e = hipSetDevice(device);
+        void* p = nullptr;
+        if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return fail(LLSV_ERR_DEVICE, "%s: scratch address on device %d: %s", who, device, hipGetErrorString(e));
+        g_scratch_dev[device] = p;
+    }
+    *out = static_cast<unsigned char*>(g_scratch_dev[device]);
+    return LLSV_OK;
+}
+
+}  // namespace
+
+extern "C" int llsv_sample(llsv_state* s, uint64_t shots, uint64_t seed, uint64_t* out) {
+    if (!s) return fail(LLSV_ERR_HANDLE, "llsv_sample: null state handle");
+    if (shots && !out) return fail(LLSV_ERR_ARG, "llsv_sample: out is null");
+    if (shots == 0) return LLSV_OK;
+    const uint64_t bytes = sample_bytes(s->size, shots);
+    unsigned char* d = nullptr;
+    hipError_t e = dev_alloc(reinterpret_cast<void**>(&d), bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(LLSV_ERR_NOMEM, "llsv_sample: cannot allocate %llu bytes", (unsigned long long)bytes);
+    }
+    const int rc = sample_into(s, shots, seed, out, d, "llsv_sample");
+    (void)hipFree(d);
+    return rc;
+}
+
+extern "C" int llsv_sample_noalloc(llsv_state* s, uint64_t shots, uint64_t seed, uint64_t* out) {
+    if (!s) return fail(LLSV_ERR_HANDLE, "llsv_sample_noalloc: null state handle");
+    if (shots && !out) return fail(LLSV_ERR_ARG, "llsv_sample_noalloc: out is null");
+    if (shots == 0) return LLSV_OK;
+    const uint64_t bytes = sample_bytes(s->size, shots);
+    if (shots > LLSV_SAMPLE_SCRATCH_BYTES / 8 || bytes > LLSV_SAMPLE_SCRATCH_BYTES)
+        return fail(LLSV_ERR_ARG,
+                    "llsv_sample_noalloc: %llu shots on %d qubits need %llu bytes of scratch, more than the %llu "
+                    "available (use llsv_sample)",
+                    (unsigned long long)shots, s->n, (unsigned long long)bytes,
+                    (unsigned long long)LLSV_SAMPLE_SCRATCH_BYTES);
+    unsigned char* d = nullptr;
+    if (int rc = scratch_on(s->device, &d, "llsv_sample_noalloc")) return rc;
+    return sample_into(s, shots, seed, out, d, "llsv_sample_noalloc");
 }

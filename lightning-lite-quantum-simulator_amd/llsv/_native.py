@@ -48,6 +48,7 @@ SIGNATURES = {
     "llsv_probabilities": (_I, [_P, _P, _U64]),
     "llsv_expect_pauli": (_I, [_P, _P, _I, _P, _P]),
     "llsv_sample": (_I, [_P, _U64, _U64, _P]),
+    "llsv_sample_noalloc": (_I, [_P, _U64, _U64, _P]),
     "llsv_copy_to_host": (_I, [_P, _P, _U64]),
     "llsv_copy_from_host": (_I, [_P, _P, _U64]),
     "llsv_sync": (_I, [_P]),
