@@ -176,20 +176,23 @@ def adjoint_leg(n=20, layers=10, batch=64):
     fwd, bwd = circ.precompile()
     t_compile = time.perf_counter() - tc
     log(f"adjoint: {fwd} forward + {bwd} backward tiled passes compiled in {t_compile:.1f} s")
-    LA.energies_and_grads(circ, th[:2])
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    e, g = LA.energies_and_grads(circ, th)
-    torch.cuda.synchronize()
-    t_adj = time.perf_counter() - t0
-    log(f"adjoint: {batch} gradients of {circ.n_params} params at {n} q in {t_adj:.3f} s")
+    def best_of_3(fn):
+        fn()  # warm-up
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return min(ts), ts, out
+
+    t_adj, adj_runs, (e, g) = best_of_3(lambda: LA.energies_and_grads(circ, th))
+    log(f"adjoint: {batch} gradients of {circ.n_params} params at {n} q in {t_adj:.3f} s (best of 3)")
     nps = 1
-    t0 = time.perf_counter()
-    ps = LA.param_shift_grads(circ, th[:nps])
-    torch.cuda.synchronize()
-    t_ps = time.perf_counter() - t0
+    t_ps, ps_runs, ps = best_of_3(lambda: LA.param_shift_grads(circ, th[:nps]))
     err = torch.max(torch.abs(ps - g[:nps])).item()
-    log(f"parameter shift: {nps} gradient in {t_ps:.3f} s, max |adjoint - shift| {err:.2e}")
+    log(f"parameter shift: {nps} gradient in {t_ps:.3f} s (best of 3), max |adjoint - shift| {err:.2e}")
     ws = circ.workspace_bytes(batch)
     return {
         "workload": f"{n}-qubit HEA, {layers} layers RY+RZ + CNOT ladder ({circ.n_params} params), "
@@ -198,6 +201,8 @@ def adjoint_leg(n=20, layers=10, batch=64):
         "adjoint_s_per_batch": round(t_adj, 4),
         "param_shift_grads_per_s": round(nps / t_ps, 4),
         "speedup_vs_param_shift": round((batch / t_adj) / (nps / t_ps), 1),
+        "timing": "warm-up then best of 3 for both; runs (s): adjoint "
+                  + ", ".join(f"{t:.4f}" for t in adj_runs) + "; parameter shift " + ", ".join(f"{t:.3f}" for t in ps_runs),
         "max_abs_grad_diff_vs_param_shift": err,
         "workspace_GiB": round(ws / 2**30, 2),
         "tiled_passes": [fwd, bwd],
@@ -320,11 +325,26 @@ def latency_leg(V3, n=12, n_ops=1000, seeds=10):
         sv.apply_program(ops)
         sv.sync()
     t_prog = (time.perf_counter() - t0) / seeds
-    log(f"latency {n}q x {n_ops} ops: per-gate {t_gate * 1e3:.2f} ms, program {t_prog * 1e3:.3f} ms per circuit")
+    # where a program's time goes: the host call (op conversion, plan, upload, launch) and the
+    # device time between events recorded on the state's stream around it
+    host, dev = [], []
+    for ops in circuits:
+        sv.reset()
+        sv.sync()
+        sv.mark(0)
+        th = time.perf_counter()
+        sv.apply_program(ops)
+        host.append(time.perf_counter() - th)
+        sv.mark(1)
+        dev.append(sv.mark_elapsed(0, 1) / 1e3)
+    t_host, t_dev = sum(host) / seeds, sum(dev) / seeds
+    log(f"latency {n}q x {n_ops} ops: per-gate {t_gate * 1e3:.2f} ms, program {t_prog * 1e3:.3f} ms per circuit "
+        f"(host call {t_host * 1e3:.3f} ms, device {t_dev * 1e3:.3f} ms)")
     del sv
     return {"workload": f"{n}-qubit complex128, {n_ops} random H / CNOT ops (seeds 0..{seeds - 1})",
             "per_gate_ms_per_circuit": round(t_gate * 1e3, 3), "per_gate_ops_per_s": round(n_ops / t_gate, 1),
-            "program_ms_per_circuit": round(t_prog * 1e3, 4), "program_ops_per_s": round(n_ops / t_prog, 1)}
+            "program_ms_per_circuit": round(t_prog * 1e3, 4), "program_ops_per_s": round(n_ops / t_prog, 1),
+            "program_host_call_ms": round(t_host * 1e3, 4), "program_device_ms": round(t_dev * 1e3, 4)}
 
 
 def zne_leg(n=12, layers=4, p=0.005, trajectories=8192):

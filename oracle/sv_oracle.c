@@ -264,3 +264,16 @@ void orc_axpy_pauli(const double* psi, int n, uint64_t xmask, uint64_t zmask, do
         out[2 * i + 1] += sg * psi[2 * src + 1];
     }
 }
+
+/* STREAM copy (McCalpin's "Copy" kernel, a[i] = b[i]) over n doubles, OpenMP static schedule,
+ * for the host-bandwidth figure reported beside the CPU baseline: 16 bytes moved per element. */
+void orc_stream_copy(double* dst, const double* src, uint64_t n) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) dst[i] = src[i];
+}
+
+/* Parallel first touch of n doubles (zeros), so pages land near the threads that use them. */
+void orc_fill_zero(double* dst, uint64_t n) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; ++i) dst[i] = 0.0;
+}
