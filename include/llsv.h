@@ -376,6 +376,9 @@ int llsv_noisy_site(llsv_noisy* z, int nq, uint32_t* code);
  * from |0...0> on n <= LLSV_ZNE_MAX_QUBITS qubits, n_gates * scale <= 2^22 gate applications (a
  * state of up to 12 qubits lives in a workgroup's LDS, a larger one in a device-memory slot of the
  * library: at most 4 GiB of slots, kept on the device for later calls until the process ends).
+ * The folded circuit's tables take up to 104 bytes of device memory per gate application (0.44 GB
+ * at the cap); they are kept for the next call while they hold at most 64 MiB, and freed at the
+ * end of a call above that.  Every gate's qubits number at most 16 (the noise code of a site).
  * weights (host, 2^n doubles, or NULL): values[t]
  * = sum_i weights[i] |psi_t[i]|^2 (host, n_traj doubles).  amps (host, n_traj * 2^n complex128
  * interleaved, or NULL): the final states.  With p = 0 no draws are made.  Replaces the

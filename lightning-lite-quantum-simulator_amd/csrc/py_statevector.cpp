@@ -155,12 +155,12 @@ public:
     // ---- readout
     double norm() {
         double v = 0.0;
-        device_call([&] { return llsv_norm2(st_, &v); });
+        read([&] { return llsv_norm2(st_, &v); });
         return v;
     }
     double expect_z(int q) {
         double v = 0.0;
-        device_call([&] { return llsv_expect_z(st_, q, &v); });
+        read([&] { return llsv_expect_z(st_, q, &v); });
         return v;
     }
     // <psi|H|psi> for H = sum of (coeff, xmask, zmask) Pauli terms
@@ -168,14 +168,14 @@ public:
         std::vector<llsv_pauli> t(terms.size());
         for (size_t i = 0; i < terms.size(); ++i) t[i] = llsv_pauli{std::get<0>(terms[i]), std::get<1>(terms[i]), std::get<2>(terms[i])};
         double total = 0.0;
-        device_call([&] { return llsv_expect_pauli(st_, t.data(), int(t.size()), nullptr, &total); });
+        read([&] { return llsv_expect_pauli(st_, t.data(), int(t.size()), nullptr, &total); });
         return total;
     }
 
     py::array_t<uint64_t> sample(uint64_t shots, uint64_t seed) {
         py::array_t<uint64_t> out(static_cast<py::ssize_t>(shots));
         uint64_t* p = out.mutable_data();
-        device_call([&] { return llsv_sample(st_, shots, seed, p); });
+        read([&] { return llsv_sample(st_, shots, seed, p); });
         return out;
     }
 
@@ -183,14 +183,14 @@ public:
         py::array_t<double> out(static_cast<py::ssize_t>(size()));
         double* p = out.mutable_data();
         const uint64_t n = size();
-        device_call([&] { return llsv_probabilities(st_, p, n); });
+        read([&] { return llsv_probabilities(st_, p, n); });
         return out;
     }
     py::array_t<cplx> state() {
         py::array_t<cplx> out(static_cast<py::ssize_t>(size()));
         double* p = reinterpret_cast<double*>(out.mutable_data());
         const uint64_t n = size();
-        device_call([&] { return llsv_copy_to_host(st_, p, n); });
+        read([&] { return llsv_copy_to_host(st_, p, n); });
         return out;
     }
 
@@ -238,6 +238,7 @@ public:
         return ms;
     }
     void copy_from(StateVector& other) {
+        other.refresh_device();
         mutate([&] { return llsv_copy_state(st_, other.st_); }, /*upload=*/false);
     }
     uintptr_t device_ptr() { return reinterpret_cast<uintptr_t>(llsv_device_ptr(st_)); }
@@ -272,6 +273,26 @@ private:
             rc = f();
         }
         check(rc);
+    }
+
+    // A call that only reads the state: while a host buffer is linked (and transfers are not
+    // deferred) the host copy is the current one -- NumPy may have written it since the last
+    // call -- so it goes to the device first.
+    template <class F>
+    void read(F&& f) {
+        const bool linked = host_linked() && !deferred_;
+        double* h = host_;
+        const uint64_t n = size();
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = linked ? llsv_copy_from_host(st_, h, n) : LLSV_OK;
+            if (rc == LLSV_OK) rc = f();
+        }
+        check(rc);
+    }
+    void refresh_device() {
+        if (host_linked() && !deferred_) push();
     }
 
     void push() {

@@ -159,6 +159,7 @@ __device__ inline void block_sums(double* v, double (*lds)[WG / 64], double* out
 }
 
 __global__ void k_traj_reset(unsigned* counter) { *counter = 0u; }
+__global__ void k_traj_set_item(int* slot, int v) { *slot = v; }
 
 // Joint weights of bits (sa, sb) of amplitude i: v[2 * bit_sa + bit_sb] += p (selects, so v stays
 // in registers).  A one-qubit statistic uses sa = sb: v[0] / v[3] get the weight of bit 0 / 1.
@@ -712,8 +713,10 @@ extern "C" int llsv_traj_run(const llsv_traj_item* items, int n_items, const lls
             if (e == hipSuccess && state) {
                 // item 0's final state is in the slot of whichever workgroup ran it last: rerun it alone.
                 e = hipStreamSynchronize(st);
-                const int zero = 0;
-                if (e == hipSuccess) e = hipMemcpyAsync(d.sample, &zero, sizeof(int), hipMemcpyHostToDevice, st);
+                if (e == hipSuccess) {
+                    hipLaunchKernelGGL(k_traj_set_item, dim3(1), dim3(1), 0, st, d.sample, 0);
+                    e = hipGetLastError();
+                }
                 if (e == hipSuccess)
                     e = launch_traj(cfg, 1, st, d.items, 1, d.sample, d.gates, d.gamma, d.lam, d.seed, d.ws, d.slot,
                                     d.counter, d.z, d.j, d.h, 1);

@@ -36,6 +36,7 @@ constexpr int kZSlotThreads = 1024;  // above 12 qubits: one trajectory per work
 constexpr int kZTileQubits = 12;     // device-memory states: runs of applications on at most 12 qubits
 constexpr int kZRunMax = 256;        // go through LDS tiles of 2^12 amplitudes, at most this many per run
 constexpr int64_t kZMaxApps = int64_t(1) << 22;  // gate applications of one folded circuit
+constexpr size_t kZKeepBytes = size_t(64) << 20;    // per-application tables kept between calls
 constexpr int kZLdsQubits = 12;  // up to 2^12 amplitudes (64 KiB) per state: 512-thread workgroups
 
 // One gate application of the folded circuit.
@@ -429,6 +430,13 @@ std::vector<ZApp> fold(const llsv_zgate* gates, int G, int folds) {
 struct ZBuf {
     void* p = nullptr;
     size_t cap = 0;
+    void release_above(size_t keep) {
+        if (cap > keep) {
+            (void)hipFree(p);
+            p = nullptr;
+            cap = 0;
+        }
+    }
     hipError_t need(size_t bytes) {
         if (bytes <= cap) return hipSuccess;
         if (p) (void)hipFree(p);
@@ -677,6 +685,9 @@ int llsv_noisy_run(llsv_noisy* z, int n, const llsv_zgate* gates, int n_gates, i
     if (weights) ZK(hipMemcpy(values, b_vals.p, sizeof(double) * N, hipMemcpyDeviceToHost));
     ZK(hipDeviceSynchronize());
 #undef ZK
+    // the per-application tables grow with the folded circuit (104 bytes per application with
+    // the renumbered copy): keep them for the next call up to kZKeepBytes, release larger ones
+    for (ZBuf* b : {&b_apps, &b_tapps, &b_run}) b->release_above(kZKeepBytes);
     if (z->rng == 1) z->philox_next += N;
     return LLSV_OK;
 }

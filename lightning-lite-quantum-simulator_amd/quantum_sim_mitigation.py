@@ -38,6 +38,8 @@ MAX_ZNE_SCALE = 99
 DEFAULT_ZNE_TRAJECTORIES = 1000
 MAX_DM_QUBITS = 10
 MAX_BATCH_QUBITS = 20  # LLSV_ZNE_MAX_QUBITS
+MAX_SITE_QUBITS = 16  # qubits of one llsv_noisy_site draw / one llsv_zgate (two code bits per qubit)
+MAX_BATCH_APPS = 1 << 22  # gate applications of one folded circuit in llsv_noisy_run
 
 
 class _NQ(int):
@@ -213,8 +215,17 @@ class NoisyBackend:
         if self._p == 0.0:
             return
         code = ctypes.c_uint32(0)
-        _native.check(self._z.lib.llsv_noisy_site(self._z.h, len(qs), ctypes.byref(code)))
-        c = code.value
+        if len(qs) <= MAX_SITE_QUBITS:
+            _native.check(self._z.lib.llsv_noisy_site(self._z.h, len(qs), ctypes.byref(code)))
+            c = code.value
+        else:
+            # a gate on >= 3 qubits gets an independent one-qubit channel per qubit
+            # (src/Mitigation.h:196-204), so drawing them one qubit at a time consumes the stream
+            # exactly as one wide draw would
+            c = 0
+            for i in range(len(qs)):
+                _native.check(self._z.lib.llsv_noisy_site(self._z.h, 1, ctypes.byref(code)))
+                c |= code.value << (2 * i)
         for i, q in enumerate(qs):
             w = (c >> (2 * i)) & 3
             if w in (2, 3):
@@ -301,8 +312,8 @@ def pack_gates(gates):
         o = arr[i]
         o.kind = _ZG[g[0]]
         qs = list(g[1]) if g[0] == "mcz" else ([g[1], g[2]] if g[0] == "cnot" else [g[1]])
-        if len(qs) > 16:
-            raise ValueError("mcz: at most 16 qubits in a noisy circuit")
+        if len(qs) > MAX_SITE_QUBITS:
+            raise ValueError(f"mcz: at most {MAX_SITE_QUBITS} qubits in one batched gate")
         o.nq = len(qs)
         for j, q in enumerate(qs):
             o.q[j] = int(q)
@@ -325,6 +336,15 @@ class _FinalState:
 
     def probabilities(self) -> np.ndarray:
         return self._a.real ** 2 + self._a.imag ** 2
+
+
+def batchable(gates, scale: int) -> bool:
+    """Whether llsv_noisy_run takes this folded circuit: every gate's noise fits one code word
+    (<= 16 qubits) and the folded circuit has <= 2^22 gate applications.  ZNE.extrapolate runs
+    other scales through the per-gate loop, which continues the same random stream."""
+    if len(gates) * int(scale) > MAX_BATCH_APPS:
+        return False
+    return all(len(g[1]) <= MAX_SITE_QUBITS for g in gates if g[0] == "mcz")
 
 
 def run_trajectories(noisy: NoisyBackend, n: int, gates, scale: int, n_traj: int, observable):
@@ -471,7 +491,7 @@ class ZNE:
             for s in out.scales:
                 scale = int(s)
                 noisy.set_scale(scale)
-                if gates is not None:
+                if gates is not None and batchable(gates, scale):
                     vals = run_trajectories(noisy, n, gates, scale, ntr, observable_fn)
                     noisy.reset(n)  # the counters of one trajectory, as after the reference's loop
                     noisy._logical, noisy._physical = len(gates), scale * len(gates)

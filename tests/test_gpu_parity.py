@@ -198,6 +198,38 @@ def test_zero_copy_and_view_semantics():
     assert m.StateVector(4).get_num_qubits() == 4 and m.StateVector(4).get_size() == 16
 
 
+def test_host_writes_seen_by_readouts():
+    # NumPy writes to a borrowed array or a live view reach every readout, not only gate calls
+    m = qsv()
+    st = np.zeros(1 << 6, dtype=np.complex128)
+    st[0] = 1
+    sv = m.StateVector(st)
+    st[0] = 0
+    st[5] = 0.6
+    st[9] = 0.8j
+    assert abs(sv.norm() - 1.0) < 1e-15
+    assert abs(sv.expect_z(0) + 1.0) < 1e-15 and abs(sv.expect_z(2) - (-0.36 + 0.64)) < 1e-15
+    p = sv.probabilities()
+    assert abs(p[5] - 0.36) < 1e-15 and abs(p[9] - 0.64) < 1e-15
+    assert np.array_equal(sv.state(), st)
+    assert abs(sv.expect_pauli([(1.0, 0, 8)]) - (0.36 - 0.64)) < 1e-15
+    assert set(sv.sample(200, 3).tolist()) <= {5, 9}
+    owned = m.StateVector(6)
+    view = owned.get_numpy_view()
+    view[0] = 0
+    view[63] = 1
+    assert abs(owned.expect_z(5) + 1.0) < 1e-15 and owned.sample(4, 1).tolist() == [63] * 4
+    other = m.StateVector(6)
+    other.copy_from(owned)  # the source's host writes travel too
+    assert other.state()[63] == 1
+    st[:] = 0
+    st[1] = 1
+    other.copy_from(sv)
+    assert other.state()[1] == 1
+    with pytest.raises(RuntimeError):  # a state with no probability cannot be sampled
+        m.StateVector(np.zeros(8, dtype=np.complex128)).sample(2, 1)
+
+
 def test_v2_surface():
     import quantum_sim_v2
 

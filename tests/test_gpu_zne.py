@@ -83,6 +83,25 @@ def test_wide_mcz_between_tiled_runs():
         assert np.max(np.abs(ours - theirs)) < 1e-12
 
 
+def test_mcz_wider_than_a_code_word():
+    # a 17-qubit mcz: one noise channel per qubit, drawn one qubit at a time; ZNE runs such a
+    # circuit through the per-gate loop and still follows the reference stream
+    import quantum_sim_mitigation as QM
+
+    n, p, seed = 17, 0.2, 33
+    gates = ([("h", q) for q in range(4)] + [("mcz", list(range(n)))] + [("cnot", 0, 16), ("rx", 16, 0.7)]
+             + [("mcz", list(range(n))[::-1])] + [("ry", 2, 0.4)])
+    assert not QM.batchable(gates, 1)
+    w = np.random.default_rng(seed).uniform(-1, 1, 2 ** n)
+    say("zne mcz on 17 qubits")
+    nb = QM.NoisyBackend(QM.ReferenceBackend(n), p, seed)
+    r = QM.ZNE(8).extrapolate(nb, lambda b: [_apply(b, g) for g in gates], QM.Diagonal(w), [1, 3])
+    o = Z.extrapolate(n, gates, p, seed, 8, [1, 3], w)
+    assert abs(r.value - o["value"]) < 1e-12 and abs(r.std_error - o["std_error"]) < 1e-12
+    assert np.max(np.abs(np.array(r.values) - o["values"])) < 1e-12
+    assert nb.logical_gates == len(gates) and nb.physical_gates == 3 * len(gates)
+
+
 def test_reference_stream_matches_compiled_reference():
     lib = ref_lib()
     if lib is None:

@@ -280,6 +280,55 @@ def _same(op, g):
     return op[1] == g[1]
 
 
+def test_product_noise_stream_wide_mcz():
+    """A 17-qubit mcz (one noise channel per qubit, drawn one qubit at a time by the product):
+    the per-gate path inserts the oracle's Paulis at the oracle's sites, and the batched path is
+    not taken for it."""
+    import quantum_sim_mitigation as QM
+
+    n, p, seed, scale = 17, 0.3, 9, 3
+    wide = list(range(n))
+    circ = [("h", 0), ("cnot", 0, 1), ("mcz", wide), ("rx", 3, 0.4), ("mcz", [2, 5, 16]), ("mcz", wide[::-1])]
+    log = _Log(n)
+    nb = QM.NoisyBackend(log, p, seed)
+    nb.set_scale(scale)
+    orc = Z.NoisyOracle(n, p, seed)
+    orc.set_scale(scale)
+    for _ in range(4):
+        nb.reset(n)
+        orc.reset()
+        for g in circ:
+            _apply(nb, g)
+            orc.gate(g)
+        expect = []
+        for g in circ:
+            expect += [g] * scale
+        pos, site, paulis = 0, -1, []
+        for op in log.ops:
+            if pos < len(expect) and _same(op, expect[pos]):
+                site += 1
+                pos += 1
+            else:
+                paulis.append((site, op))
+        assert pos == len(expect)
+        codes = {}
+        for s_, op in paulis:
+            codes.setdefault(s_, []).append(op)
+        assert sorted(codes) == [s_ for s_, _ in orc.events]
+        for s_, code in orc.events:
+            want = []
+            for j, q in enumerate(Z.gate_qubits(expect[s_])):
+                w = (code >> (2 * j)) & 3
+                if w in (2, 3):
+                    want.append(("mcz", (q,)))
+                if w in (1, 2):
+                    want.append(("x", q))
+            assert codes[s_] == want
+    assert any(len(Z.gate_qubits(expect[s_])) == n for s_, _ in orc.events)
+    assert not QM.batchable(circ, 1) and QM.batchable(circ[:2] + circ[3:5], 99)
+    assert not QM.batchable([("h", 0)] * (1 << 20), 5) and QM.batchable([("h", 0)] * (1 << 20), 3)
+
+
 def test_product_host_validation_and_weights():
     import quantum_sim_mitigation as QM
 
