@@ -46,6 +46,25 @@ __device__ __forceinline__ uint64_t tile_offset(const PassDesc& d, uint32_t l) {
     return o;
 }
 
+// The fields of a PassOp that every kind but P_DENSE uses.
+struct OpHead {
+    int kind, a, b;
+    uint64_t lmask, omask;
+    double m[8];
+};
+
+__device__ __forceinline__ OpHead op_head(const PassOp& op) {
+    OpHead h;
+    h.kind = op.kind;
+    h.a = op.a;
+    h.b = op.b;
+    h.lmask = op.lmask;
+    h.omask = op.omask;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) h.m[j] = op.m[j];
+    return h;
+}
+
 // DENSE selects the variant that can run dense blocks (it holds a column of outputs in
 // registers, so the plain variant stays lean for passes without one).
 template <int NT, bool DENSE>
@@ -68,8 +87,12 @@ __global__ __launch_bounds__(NT) void k_pass(v2d* __restrict__ psi, PassDesc d, 
     __syncthreads();
 
     double sr = 1.0, si = 0.0;  // accumulated per-tile scalar (phases of qubits outside T)
+    // The op's fields are read while the previous op runs: a one-tile state runs its whole
+    // circuit in this loop, and a read of the op list on the critical path costs as much as the op.
+    OpHead next = d.nops > 0 ? op_head(ops[0]) : OpHead{};
     for (int i = 0; i < d.nops; ++i) {
-        const PassOp& op = ops[i];
+        const OpHead op = next;
+        if (i + 1 < d.nops) next = op_head(ops[i + 1]);
         const int kind = op.kind;
         if (kind == P_SCALE) {
             const bool hi = (base >> op.a) & 1;
@@ -108,10 +131,11 @@ __global__ __launch_bounds__(NT) void k_pass(v2d* __restrict__ psi, PassDesc d, 
                     if ((l & lm) == lm) tile[l] = -tile[l];
             }
         } else if (DENSE && kind == P_DENSE) {
-            const int k = op.a, D = 1 << k;
+            const PassOp& dop = ops[i];
+            const int k = dop.a, D = 1 << k;
             uint32_t tmask = 0;
-            for (int j = 0; j < k; ++j) tmask |= 1u << op.q[j];
-            const v2d* U = mats + op.moff;
+            for (int j = 0; j < k; ++j) tmask |= 1u << dop.q[j];
+            const v2d* U = mats + dop.moff;
             constexpr int kMaxPer = (1 << kMaxTileBits) / NT;
             v2d out[kMaxPer];
 #pragma unroll
@@ -119,12 +143,12 @@ __global__ __launch_bounds__(NT) void k_pass(v2d* __restrict__ psi, PassDesc d, 
                 const uint32_t o = tid + r * NT;
                 if (o < tsize) {
                     uint32_t row = 0;
-                    for (int j = 0; j < k; ++j) row |= ((o >> op.q[j]) & 1u) << j;
+                    for (int j = 0; j < k; ++j) row |= ((o >> dop.q[j]) & 1u) << j;
                     const uint32_t gb = o & ~tmask;
                     const v2d* ur = U + row * D;
                     v2d acc = {0.0, 0.0};
                     for (int c = 0; c < D; ++c) {
-                        const v2d u = ur[c], x = tile[gb | op.dep[c]];
+                        const v2d u = ur[c], x = tile[gb | dop.dep[c]];
                         acc.x += u.x * x.x - u.y * x.y;
                         acc.y += u.x * x.y + u.y * x.x;
                     }
@@ -152,6 +176,7 @@ __global__ __launch_bounds__(NT) void k_pass(v2d* __restrict__ psi, PassDesc d, 
 }
 
 constexpr int kPassThreads = 256;
+constexpr int kSingleThreads = 1024;  // a one-tile state: 16 waves hide the LDS latency of each op
 
 // ---------------------------------------------------------------- register-resident tiles
 // The tile (2^12 amplitudes) lives in registers, 16 per thread: 4 "register bits" of the
@@ -707,14 +732,17 @@ struct BuiltProgram {
 };
 
 // Host half of llsv_apply_program: validation, the pass schedule and every pass's op list.
-int build_program(int n, const llsv_op* ops, uint64_t n_ops, int tile_bits, BuiltProgram* out) {
+// single_lds: a state that is one tile (n <= tile bits) runs its pass LDS-resident (k_pass with
+// kSingleThreads threads) instead of in register tiles.
+int build_program(int n, const llsv_op* ops, uint64_t n_ops, int tile_bits, BuiltProgram* out,
+                  bool single_lds = false) {
     std::vector<Checked> info(n_ops);
     for (uint64_t i = 0; i < n_ops; ++i)
         if (int rc = check_op(n, ops[i], i, &info[i])) return rc;
 
     const int b = std::min(n, tile_bits);
     const int L = std::min(kLowBits, b);
-    const bool reg_tiles = (b == kRegTileBits);
+    const bool reg_tiles = (b == kRegTileBits) && !(single_lds && n == b);
     out->b = b;
     out->reg_tiles = reg_tiles;
     const std::vector<PlannedPass> passes = plan(n, ops, info, b, L);
@@ -848,7 +876,11 @@ int build_program(int n, const llsv_op* ops, uint64_t n_ops, int tile_bits, Buil
 int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
     const int n = s->n;
     BuiltProgram bp;
-    if (int rc = build_program(n, ops, n_ops, device_tile_bits(s), &bp)) return rc;
+    // A state that fits one tile is latency-bound (one workgroup does all the work): there the
+    // LDS-resident pass with 1024 threads, one barrier per op, beats register tiles, whose ops on
+    // lane bits each need a full exchange through LDS.  In the automatic mode only: mode 0 runs the
+    // register-tile interpreter and mode 2 compiles register tiles (the two agree bit for bit).
+    if (int rc = build_program(n, ops, n_ops, device_tile_bits(s), &bp, jit_mode() == kJitAuto)) return rc;
     const int b = bp.b;
     const bool reg_tiles = bp.reg_tiles;
 
@@ -916,6 +948,7 @@ int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
         } else {
             const uint64_t tiles = uint64_t(1) << (n - b);
             const dim3 grid{unsigned(tiles), 1, 1}, block{unsigned(kPassThreads), 1, 1};
+            const bool single = !reg_tiles && n == b;
             if (jit_ready(jit[li])) {
                 // a pass whose kernel is not ready (compiling, or failed to compile) is interpreted
                 e = jit_launch(jit[li], unsigned(tiles), s->stream, s->amps, la.d, drops + la.first);
@@ -924,6 +957,15 @@ int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
                     hipLaunchKernelGGL((k_pass_reg<true>), grid, block, lds, s->stream, s->amps, la.d, drops + la.first, dmats);
                 else
                     hipLaunchKernelGGL((k_pass_reg<false>), grid, block, lds, s->stream, s->amps, la.d, drops + la.first, dmats);
+                e = hipGetLastError();
+            } else if (single) {
+                const dim3 sblock{unsigned(kSingleThreads), 1, 1};
+                if (la.dense)
+                    hipLaunchKernelGGL((k_pass<kSingleThreads, true>), grid, sblock, lds, s->stream, s->amps, la.d,
+                                       dops + la.first, dmats);
+                else
+                    hipLaunchKernelGGL((k_pass<kSingleThreads, false>), grid, sblock, lds, s->stream, s->amps, la.d,
+                                       dops + la.first, dmats);
                 e = hipGetLastError();
             } else if (la.dense) {
                 hipLaunchKernelGGL((k_pass<kPassThreads, true>), grid, block, lds, s->stream, s->amps, la.d,
