@@ -140,6 +140,34 @@ def layered_leg(V3, n=28, layers=20):
     t_fused = time.perf_counter() - t0
     k5 = sum(1 for o in fused if o[0] == "dense" and len(o[1]) == 5)
     log(f"layered {n}q fused: {len(fused)} blocks ({k5} of 5 qubits), {t_fused:.3f} s")
+
+    # the fusion cost model (gate_fusion.plan_fusion): dense blocks only where predicted to pay
+    def timed(fn):
+        fn()
+        sv.sync()
+        t0 = time.perf_counter()
+        fn()
+        sv.sync()
+        return time.perf_counter() - t0
+
+    segs, t_pred = GF.plan_fusion(n, ops)
+    t_auto = timed(lambda: GF.run_fusion_plan(sv, segs))
+    # a deep circuit on five of the 28 qubits (300 rotations and CNOTs): one register-tile pass is
+    # FP64-bound there, one dense 5-qubit block is one HBM pass on the matrix cores
+    rng = np.random.Generator(np.random.PCG64(5))
+    qs = [3, 7, 11, 19, 25]
+    deep = []
+    for _ in range(300):
+        if rng.random() < 0.7:
+            deep.append((("rx", "ry", "rz")[int(rng.integers(3))], int(rng.choice(qs)), float(rng.uniform(0, 2 * np.pi))))
+        else:
+            c, t = rng.choice(qs, 2, replace=False)
+            deep.append(("cnot", int(c), int(t)))
+    dsegs, d_pred = GF.plan_fusion(n, deep)
+    t_deep_prog = timed(lambda: sv.apply_program(deep))
+    t_deep_auto = timed(lambda: GF.run_fusion_plan(sv, dsegs))
+    log(f"layered {n}q fusion plan: {[k for k, _ in segs]} {t_auto:.3f} s; deep 5-qubit circuit: program "
+        f"{t_deep_prog * 1e3:.2f} ms, plan {[k for k, _ in dsegs]} {t_deep_auto * 1e3:.2f} ms")
     del sv
     return {
         "workload": f"{n}-qubit complex128 layered circuit, {layers} layers of RX/RY/RZ on every qubit + CZ brick "
@@ -156,6 +184,11 @@ def layered_leg(V3, n=28, layers=20):
                        "hbm_GBps": round(32.0 * size * len(fused) / (dev_ms / 1e3) / 1e9, 1),
                        "TFLOPs": round(flops / (dev_ms / 1e3) / 1e12, 2),
                        "mfma_fraction": round(flops / (dev_ms / 1e3) / 78.6e12, 4)},
+        "fused_auto": {"s": round(t_auto, 4), "segments": [k for k, _ in segs], "predicted_s": round(t_pred, 4),
+                       "gates_per_s": round(len(ops) / t_auto, 1)},
+        "deep5": {"what": "300 rotations and CNOTs on qubits 3, 7, 11, 19, 25 of the same 28-qubit state",
+                  "program_s": round(t_deep_prog, 5), "fused_auto_s": round(t_deep_auto, 5),
+                  "segments": [k for k, _ in dsegs], "predicted_s": round(d_pred, 5)},
     }
 
 
@@ -512,6 +545,17 @@ def q32_leg(V3, n=32, layers=12):
     t_f = time.perf_counter() - t0
     norm = sv.norm()
     log(f"32q fused: {len(fused)} blocks in {t_f:.3f} s, norm-1 {norm - 1:.1e}")
+    # the same circuit through the fusion cost model (dense blocks only where they are predicted to pay)
+    segs, t_pred = GF.plan_fusion(n, lay)
+    sv.reset()
+    V3.program_jit_precompile(n, [o for k, b in segs if k == "program" for o in b])
+    GF.run_fusion_plan(sv, segs)
+    sv.sync()
+    t0 = time.perf_counter()
+    GF.run_fusion_plan(sv, segs)
+    sv.sync()
+    t_auto = time.perf_counter() - t0
+    log(f"32q fusion plan: {[k for k, _ in segs]} in {t_auto:.3f} s (predicted {t_pred:.3f} s)")
     del sv
     return {
         "workload": f"{n}-qubit complex128 (64 GiB): Haar + CNOT-ladder sweep ({len(ops)} gates) and a {layers}-layer "
@@ -522,6 +566,8 @@ def q32_leg(V3, n=32, layers=12):
                        "mfma_fraction": round(flops / (dev_ms / 1e3) / 78.6e12, 4),
                        "hbm_GBps": round(32.0 * size * len(fused) / (dev_ms / 1e3) / 1e9, 1),
                        "gates_per_s": round(len(lay) / t_f, 1)},
+        "fused_auto": {"s": round(t_auto, 4), "segments": [k for k, _ in segs], "predicted_s": round(t_pred, 4),
+                       "gates_per_s": round(len(lay) / t_auto, 1)},
         "norm_minus_1": norm - 1,
     }
 

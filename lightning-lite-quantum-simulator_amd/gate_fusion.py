@@ -17,6 +17,13 @@ or ``('u', q, g4)`` for one-qubit blocks.  Blocks are grown greedily along the c
 an op joins the open block while the union of qubits stays within ``max_k``; otherwise
 the block is emitted and a new one opens.  The emitted circuit equals the input one up to
 rounding.
+
+``plan_fusion(n, ops)`` chooses, region by region, between the two ways the engine runs a
+stretch of circuit: one register-tile pass of the program path (``apply_program``: one read
+and write of the state, every gate in FP64 vector arithmetic) or dense blocks of at most five
+qubits (``apply_fused``: one read and write per block, the arithmetic on the FP64 matrix
+cores).  The regions are the program planner's passes; each gets the option with the lower
+predicted time (``predict_pass`` / ``predict_blocks``); ``run_fusion_plan`` executes the result.
 """
 from __future__ import annotations
 
@@ -24,7 +31,8 @@ import math
 
 import numpy as np
 
-__all__ = ["GateFuser", "GATES", "fuse_dense", "op_matrix"]
+__all__ = ["GateFuser", "GATES", "fuse_dense", "op_matrix", "plan_fusion", "run_fusion_plan", "predict_pass",
+           "predict_blocks", "predict_program"]
 
 
 class GateFuser:
@@ -181,3 +189,147 @@ def fuse_dense(ops, max_k: int = 5):
         M = op_matrix(op, targets) @ M
     flush()
     return out
+
+
+# ---------------------------------------------------------------- cost model: dense blocks or a register pass
+# Rates of one MI355X, measured by the benchmark legs (bench.py): a streamed pass over the state
+# (read and write, 32 B per amplitude) at the one-qubit kernels' 6.2 TB/s; FP64 vector instructions
+# of a register-tile pass at 3.93e13 per second (78.6 TFLOP/s of FMAs); a dense block of k qubits
+# at the bandwidth its kernel reaches (k = 4, 5: FP64 matrix cores, 5.1 / 4.8 TB/s; k <= 3:
+# vector arithmetic).  A register-tile pass costs its memory time plus its FP64 issue time (the
+# two overlap little at two workgroups per CU): this reproduces the 30-qubit sweep's passes (7.7 ms
+# predicted, 7.5 measured) and the 28-qubit layered circuit's (3.2 vs 3.8 ms).
+PASS_BPS = 6.2e12
+FP64_INSTR_PER_S = 3.93e13
+BLOCK_BPS = {1: 6.2e12, 2: 5.5e12, 3: 5.0e12, 4: 5.1e12, 5: 4.8e12}
+# FP64 vector instructions per amplitude for the ops of a register-tile pass: a general 2x2
+# (RX RY RZ merged, Haar, H) 8; an RX or RY alone 4; a diagonal one-qubit gate 2; CNOT, CZ and
+# MCZ move or negate amplitudes (about 0.5).
+_OPS_GENERAL, _OPS_ROT, _OPS_DIAG, _OPS_PERM = 8.0, 4.0, 2.0, 0.5
+
+
+def _fp64_per_amp(ops) -> float:
+    """FP64 vector instructions per amplitude of `ops` in a register-tile pass.  One-qubit gates on
+    the same qubit with no other gate on it in between are merged into one 2x2, as the planner does."""
+    fp = 0.0
+    run = {}  # qubit -> kinds of its current one-qubit run
+
+    def close(q):
+        nonlocal fp
+        kinds = run.pop(q, None)
+        if not kinds:
+            return
+        if len(kinds) > 1 or kinds[0] in ("h", "x", "u"):
+            fp += _OPS_GENERAL
+        elif kinds[0] in ("rx", "ry"):
+            fp += _OPS_ROT
+        else:
+            fp += _OPS_DIAG
+
+    for op in ops:
+        qs = op_qubits(op)
+        if op[0] in ("h", "x", "u", "rx", "ry", "rz"):
+            run.setdefault(qs[0], []).append(op[0])
+            continue
+        for q in qs:
+            close(q)
+        fp += _OPS_GENERAL * (1 << len(qs)) if op[0] == "dense" else _OPS_PERM
+    for q in list(run):
+        close(q)
+    return fp
+
+
+def predict_pass(n: int, ops) -> float:
+    """Seconds for one register-tile pass of `ops` on an n-qubit state: its memory time plus its
+    FP64 issue time."""
+    amps = float(1 << n)
+    return 32.0 * amps / PASS_BPS + _fp64_per_amp(ops) * amps / FP64_INSTR_PER_S
+
+
+def predict_blocks(n: int, blocks) -> float:
+    """Seconds for dense blocks (fuse_dense output) on an n-qubit state: one pass per block."""
+    amps = float(1 << n)
+    t = 0.0
+    for b in blocks:
+        k = len(b[1]) if b[0] == "dense" else len(op_qubits(b))
+        t += 32.0 * amps / BLOCK_BPS[min(max(k, 1), 5)]
+    return t
+
+
+def predict_program(n: int, ops, tile_bits: int = 12) -> float:
+    """Seconds for `ops` through the program path: the planner's passes, each as predict_pass."""
+    from llsv import program as P  # the planner (host only, through libllsv)
+
+    if not ops:
+        return 0.0
+    order, ends = P.plan(n, ops, min(n, tile_bits))
+    t, start = 0.0, 0
+    for end in ends:
+        t += predict_pass(n, [ops[i] for i in order[start:end]])
+        start = end
+    return t
+
+
+def _groups(ops, max_k):
+    """The runs fuse_dense would turn into one block: (targets, ops), along the circuit."""
+    out, targets, cur = [], [], []
+    for op in ops:
+        qs = list(dict.fromkeys(op_qubits(op)))
+        if len(qs) > max_k:
+            if cur:
+                out.append((targets, cur))
+            out.append((qs, [op]))
+            targets, cur = [], []
+            continue
+        union = targets + [q for q in qs if q not in targets]
+        if cur and len(union) > max_k:
+            out.append((targets, cur))
+            targets, cur, union = [], [], qs
+        targets, cur = union, cur + [op]
+    if cur:
+        out.append((targets, cur))
+    return out
+
+
+def plan_fusion(n: int, ops, max_k: int = 5, tile_bits: int = 12):
+    """Segments [("program", ops) | ("dense", blocks)] that apply the circuit, and their predicted
+    seconds.  The circuit is cut into fuse_dense's groups (at most max_k qubits each); a group of two
+    or more qubits becomes one dense block when its FP64 work inside a register-tile pass would take
+    longer than a dense pass over the state, and the other groups stay gates for the program path
+    (consecutive ones in one segment, which the planner packs into passes).  If the program path
+    alone is predicted faster than that plan, the plan is the whole circuit as one program."""
+    segs = []
+    for targets, g in _groups(ops, max_k):
+        k = len(targets)
+        dense = (2 <= k <= 5 and len(g) > 1 and g[0][0] != "mcz"
+                 and _fp64_per_amp(g) / FP64_INSTR_PER_S > 32.0 / BLOCK_BPS[k])
+        if dense:
+            block = fuse_dense(g, max_k)
+            if segs and segs[-1][0] == "dense":
+                segs[-1][1].extend(block)
+            else:
+                segs.append(("dense", block))
+        elif segs and segs[-1][0] == "program":
+            segs[-1][1].extend(g)
+        else:
+            segs.append(("program", list(g)))
+    t_plan = sum(predict_program(n, b, tile_bits) if k == "program" else predict_blocks(n, b) for k, b in segs)
+    t_prog = predict_program(n, ops, tile_bits)
+    if t_prog <= t_plan or not any(k == "dense" for k, _ in segs):
+        return [("program", list(ops))], t_prog
+    return segs, t_plan
+
+
+def run_fusion_plan(sv, segments) -> None:
+    """Applies plan_fusion's segments to a quantum_sim_v3 StateVector, in order."""
+    for kind, body in segments:
+        if kind == "program":
+            sv.apply_program(body)
+            continue
+        for b in body:
+            if b[0] == "dense":
+                sv.apply_fused(list(b[1]), np.ascontiguousarray(b[2]))
+            elif b[0] == "u":
+                sv.apply_gate_cache_optimized(b[1], list(b[2]))
+            else:
+                sv.apply_program([b])

@@ -61,3 +61,50 @@ def test_fuse_dense_layered_block_count():
     fused = GF.fuse_dense(ops, 5)
     assert len(fused) < len(ops) // 4
     assert np.max(np.abs(O.run(10, fused) - O.run(10, ops))) < 1e-12
+
+
+# ---------------------------------------------------------------- cost model
+def deep_few_qubits(seed, qs, count):
+    rng = np.random.default_rng(seed)
+    ops = []
+    for _ in range(count):
+        if rng.random() < 0.7:
+            ops.append((("rx", "ry", "rz")[int(rng.integers(3))], int(rng.choice(qs)), float(rng.uniform(0, 6.28))))
+        else:
+            c, t = rng.choice(qs, 2, replace=False)
+            ops.append(("cnot", int(c), int(t)))
+    return ops
+
+
+def _segments_as_ops(segs):
+    out = []
+    for kind, body in segs:
+        out.extend(body)
+    return out
+
+
+def test_plan_fusion_choices_and_equivalence():
+    import gate_fusion as GF
+    from llsv import circuits as C
+    from oracle import oracle as O
+
+    # a deep circuit on five qubits of a large state: one dense block beats one FP64-heavy pass
+    deep = deep_few_qubits(5, [3, 7, 11, 19, 25], 300)
+    segs, t = GF.plan_fusion(28, deep)
+    assert [k for k, _ in segs] == ["dense"] and len(segs[0][1]) == 1
+    assert t < GF.predict_pass(28, deep)
+    # a layered circuit: register passes (one read and write per pass, many gates each)
+    lay = C.layered_rotations(1, 28, 20)
+    segs, t = GF.plan_fusion(28, lay)
+    assert [k for k, _ in segs] == ["program"] and len(segs[0][1]) == len(lay)
+    assert t < GF.predict_blocks(28, GF.fuse_dense(lay, 5))
+    # mixed, on 8 qubits where the oracle runs it: the segments apply the same circuit
+    n = 8
+    mixed = deep_few_qubits(6, [0, 2, 5], 120) + C.layered_rotations(2, n, 2) + deep_few_qubits(7, [1, 4, 6, 7], 90)
+    segs, t = GF.plan_fusion(n, mixed)
+    assert {k for k, _ in segs} == {"dense", "program"} and t <= GF.predict_program(n, mixed)
+    got = O.run(n, _segments_as_ops(segs))
+    assert np.max(np.abs(got - O.run(n, mixed))) < 1e-12
+    # the predicted pass time: memory plus FP64 issue, same-qubit one-qubit runs merged
+    assert GF.predict_pass(20, [("rx", 0, 0.1), ("ry", 0, 0.2), ("cz", 0, 1), ("rz", 3, 0.4)]) == \
+        32.0 * 2**20 / GF.PASS_BPS + (8 + 0.5 + 2) * 2**20 / GF.FP64_INSTR_PER_S

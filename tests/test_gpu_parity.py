@@ -306,6 +306,23 @@ def test_fused_blocks_mfma(n, max_k):
     assert maxdiff(gpu_run(n, fused, program=True), ref) <= TOL
 
 
+def test_fusion_plan_runs_the_circuit():
+    # the cost model's mixed plan (a dense block where a deep stretch sits on few qubits, register
+    # passes elsewhere), executed segment by segment, is the circuit
+    import gate_fusion as GF
+    from llsv import circuits as C
+    from test_fusion import deep_few_qubits
+
+    n = 16
+    mixed = deep_few_qubits(6, [0, 2, 5], 120) + C.layered_rotations(2, n, 2) + deep_few_qubits(7, [1, 4, 6, 7], 90)
+    segs, _ = GF.plan_fusion(n, mixed)
+    assert {k for k, _ in segs} == {"dense", "program"}
+    say("fusion plan n=16")
+    sv = qsv().StateVector(n)
+    GF.run_fusion_plan(sv, segs)
+    assert maxdiff(sv.state(), O.run(n, mixed)) <= TOL
+
+
 @pytest.mark.parametrize("targets", [[0, 1, 2, 3, 4], [4, 3, 2, 1, 0], [19, 2, 11, 0, 7], [15, 16, 17, 18, 19],
                                      [5, 9, 13, 17], [0, 19, 1, 18]])
 def test_dense_mfma_target_orders(targets):
