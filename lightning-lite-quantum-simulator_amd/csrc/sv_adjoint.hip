@@ -467,9 +467,11 @@ struct TiledPlan {
 // Schedule of the gate list as register-tile passes: ops in execution order; a one-qubit gate
 // becomes P_1QT on its coefficient (the inverse one in the backward sweep, preceded there by
 // P_GRAD for a rotation), CNOT and CZ as in the program path.
+// order / pass_end: the schedule (planned here when empty; the backward sweep gets the forward
+// schedule reversed, which is valid for the reversed gate list and has as few passes).
 std::vector<TiledPass> tiled_passes(int n, const std::vector<const llsv_pgate*>& seq, const std::vector<int>& coef_of,
                                     const std::vector<int>& slot_of, const std::vector<int>& gate_index, int ncoef,
-                                    bool backward) {
+                                    bool backward, std::vector<uint64_t>* order_io, std::vector<uint64_t>* end_io) {
     const int G = int(seq.size());
     std::vector<llsv_op> ops(G);
     for (int k = 0; k < G; ++k) {
@@ -489,8 +491,15 @@ std::vector<TiledPass> tiled_passes(int n, const std::vector<const llsv_pgate*>&
         }
     }
     const int b = kRegTileBits, L = 3;
-    std::vector<uint64_t> order(static_cast<size_t>(G)), pass_end(static_cast<size_t>(G) + 1);
-    const int np = plan_order(n, ops.data(), uint64_t(G), b, L, order.data(), pass_end.data());
+    std::vector<uint64_t>& order = *order_io;
+    std::vector<uint64_t>& pass_end = *end_io;
+    if (order.empty()) {
+        order.assign(static_cast<size_t>(G), 0);
+        pass_end.assign(static_cast<size_t>(G) + 1, 0);
+        const int np = plan_order(n, ops.data(), uint64_t(G), b, L, order.data(), pass_end.data());
+        pass_end.resize(size_t(std::max(np, 0)));
+    }
+    const int np = int(pass_end.size());
     std::vector<TiledPass> out;
     uint64_t first = 0;
     for (int p = 0; p < np; ++p) {
@@ -594,10 +603,19 @@ TiledPlan* tiled_plan(int n, const llsv_pgate* gates, int G, const std::vector<i
                 seq[k] = &gates[k];
                 idx[k] = k;
             }
-            plan->fwd = tiled_passes(n, seq, coef_of, slot_of, idx, plan->ncoef, false);
+            std::vector<uint64_t> forder, fend;
+            plan->fwd = tiled_passes(n, seq, coef_of, slot_of, idx, plan->ncoef, false, &forder, &fend);
+            // the backward sweep: the forward passes last to first, each one's gates last to first
+            // (positions in the reversed gate list)
+            std::vector<uint64_t> border, bend;
+            for (size_t p = fend.size(); p-- > 0;) {
+                const uint64_t start = p ? fend[p - 1] : 0;
+                for (uint64_t i = fend[p]; i-- > start;) border.push_back(uint64_t(G - 1) - forder[i]);
+                bend.push_back(border.size());
+            }
             std::reverse(seq.begin(), seq.end());
             std::reverse(idx.begin(), idx.end());
-            plan->bwd = tiled_passes(n, seq, coef_of, slot_of, idx, plan->ncoef, true);
+            plan->bwd = tiled_passes(n, seq, coef_of, slot_of, idx, plan->ncoef, true, &border, &bend);
             it = tiled_cache().emplace(key, std::move(plan)).first;
         }
         tp = it->second.get();

@@ -28,7 +28,7 @@ enum JitMode : int { kJitOff = 0, kJitAuto = 1, kJitAlways = 2 };
 // Mode kJitAlways compiles every eligible pass before its first launch.
 constexpr int kJitMinQubits = 20;
 // Passes with more ops than this keep the interpreter (compile time grows with the length).
-constexpr int kJitMaxOps = 256;
+constexpr int kJitMaxOps = 1024;
 
 int jit_mode();
 int set_jit_mode(int mode);

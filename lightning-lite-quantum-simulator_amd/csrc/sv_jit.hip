@@ -145,13 +145,25 @@ Swz choose_swizzle(const RegOp* ops, int nops) {
 
 struct Src {
     std::string s;
+    // Appends one formatted line of any length (a pass's list of gradient slots can run to
+    // thousands of characters).
     void line(const char* fmt, ...) __attribute__((format(printf, 2, 3))) {
         char buf[512];
         va_list ap;
         va_start(ap, fmt);
-        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_list ap2;
+        va_copy(ap2, ap);
+        const int len = vsnprintf(buf, sizeof buf, fmt, ap);
         va_end(ap);
-        s += buf;
+        if (len >= int(sizeof buf)) {
+            std::string big(size_t(len) + 1, '\0');
+            vsnprintf(&big[0], big.size(), fmt, ap2);
+            big.resize(size_t(len));
+            s += big;
+        } else if (len > 0) {
+            s += buf;
+        }
+        va_end(ap2);
         s += '\n';
     }
 };
