@@ -16,6 +16,10 @@
 
 #include <algorithm>
 #include <cstring>
+#include <list>
+#include <memory>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "sv_dense.h"
@@ -429,7 +433,8 @@ int popc(uint64_t x) { return __builtin_popcountll(x); }
 // move only past ops they share no qubit with.  T grows as ops join, so which ops join first
 // decides it: the planner tries a scan seeded at the first pending op of every qubit (and at the
 // first pending op overall, which keeps circuit order when everything fits), each swept to a
-// fixpoint over a window of pending ops, and keeps the seed that schedules the most ops.  On a
+// fixpoint over a window of pending ops, once with T holding only the low bits and once with the
+// seed op's qubits in T from the start, and keeps the candidate that schedules the most ops.  On a
 // 20-qubit hardware-efficient ansatz (10 layers) this gives 10 passes where a single in-order scan
 // gives 17; on the 28-qubit layered circuit (20 layers) 11 instead of 16.
 std::vector<PlannedPass> plan(int n, const llsv_op* ops, const std::vector<Checked>& info, int b, int L) {
@@ -480,10 +485,18 @@ std::vector<PlannedPass> plan(int n, const llsv_op* ops, const std::vector<Check
         seeds[0] = 0;  // position of `first` in win
         uint64_t bestT = 0;
         best.clear();
-        for (uint64_t sp : seeds) {
+        for (size_t si = 0; si < 2 * seeds.size(); ++si) {
+            const uint64_t sp = seeds[si / 2];
             ++stamp;
             cand.clear();
             uint64_t T = (b >= n) ? all : ((uint64_t(1) << L) - 1);
+            // odd candidates start with the seed op's qubits in T, so that ops which are ready
+            // earlier in the window cannot fill the tile before the seed's own chain runs
+            if (si & 1) {
+                const Checked& c = info[win[sp]];
+                if (c.diag || popc(T | c.mask) > b || (c.mask & ~T) == 0) continue;
+                T |= c.mask;
+            }
             for (int sweep = 0; sweep < kSweeps; ++sweep) {
                 bool changed = false;
                 for (uint64_t k = 0; k < win.size(); ++k) {
@@ -583,6 +596,76 @@ int plan_order(int n, const llsv_op* ops, uint64_t n_ops, int b, int L, uint64_t
 }
 
 static constexpr int kLowBits = 3;
+
+namespace {
+
+// ---- plan cache --------------------------------------------------------------------------------
+// The schedule depends only on the circuit's structure (op kinds, qubits, which one-qubit matrices
+// are diagonal), not on the matrix values, so a circuit applied again with new angles reuses it.
+// A register-tile schedule also picks its tile's low bits: tiles are made of contiguous runs of
+// 2^L amplitudes (16 << L bytes), and runs of 256 or 512 bytes stream faster than 128-byte ones
+// (a 30-qubit one-qubit + CNOT-ladder pass: 5.5 against 4.2 TB/s), but every low bit is a tile
+// bit that no gate can use.  The schedule keeps the largest L in [3, 5] that needs no more passes
+// than L = 3 does.
+struct PlanEntry {
+    int L = 0;
+    std::vector<PlannedPass> passes;
+};
+
+constexpr size_t kPlanCacheEntries = 16;
+std::mutex g_plan_mu;
+std::list<std::pair<std::string, std::shared_ptr<const PlanEntry>>>& plan_cache() {
+    static auto* c = new std::list<std::pair<std::string, std::shared_ptr<const PlanEntry>>>();  // most recent first
+    return *c;
+}
+
+std::string plan_key(int n, int b, bool reg_tiles, const llsv_op* ops, const std::vector<Checked>& info) {
+    std::string k;
+    k.reserve(8 + info.size() * 6);
+    k += char(n);
+    k += char(b);
+    k += char(reg_tiles);
+    for (size_t i = 0; i < info.size(); ++i) {
+        const llsv_op& op = ops[i];
+        const int nq = op.kind == LLSV_OP_1Q ? 1 : (op.kind == LLSV_OP_CNOT || op.kind == LLSV_OP_CZ) ? 2 : op.nq;
+        k += char(op.kind);
+        k += char(nq | (info[i].diag ? 0x40 : 0));
+        for (int t = 0; t < nq; ++t) k += char(op.q[t]);
+    }
+    return k;
+}
+
+std::shared_ptr<const PlanEntry> cached_plan(int n, int b, bool reg_tiles, const llsv_op* ops,
+                                             const std::vector<Checked>& info) {
+    const std::string key = plan_key(n, b, reg_tiles, ops, info);
+    {
+        std::lock_guard<std::mutex> lock(g_plan_mu);
+        auto& c = plan_cache();
+        for (auto it = c.begin(); it != c.end(); ++it)
+            if (it->first == key) {
+                c.splice(c.begin(), c, it);
+                return c.front().second;
+            }
+    }
+    auto e = std::make_shared<PlanEntry>();
+    e->L = std::min(kLowBits, b);
+    e->passes = plan(n, ops, info, b, e->L);
+    if (reg_tiles && n > b) {
+        for (int L = kLowBits + 1; L <= 5; ++L) {
+            std::vector<PlannedPass> alt = plan(n, ops, info, b, L);
+            if (alt.size() > e->passes.size()) break;
+            e->L = L;
+            e->passes.swap(alt);
+        }
+    }
+    std::lock_guard<std::mutex> lock(g_plan_mu);
+    auto& c = plan_cache();
+    c.emplace_front(key, e);
+    if (c.size() > kPlanCacheEntries) c.pop_back();
+    return e;
+}
+
+}  // namespace
 
 namespace {
 
@@ -803,11 +886,12 @@ int build_program(int n, const llsv_op* ops, uint64_t n_ops, int tile_bits, Buil
         if (int rc = check_op(n, ops[i], i, &info[i])) return rc;
 
     const int b = std::min(n, tile_bits);
-    const int L = std::min(kLowBits, b);
     const bool reg_tiles = (b == kRegTileBits) && !(single_lds && n == b);
     out->b = b;
     out->reg_tiles = reg_tiles;
-    const std::vector<PlannedPass> passes = plan(n, ops, info, b, L);
+    const std::shared_ptr<const PlanEntry> pe = cached_plan(n, b, reg_tiles, ops, info);
+    const int L = pe->L;
+    const std::vector<PlannedPass>& passes = pe->passes;
     out->launches.reserve(passes.size());
     for (const PlannedPass& p : passes) {
         Launch la{};
@@ -1096,8 +1180,19 @@ extern "C" int llsv_plan_passes(int n_qubits, const llsv_op* ops, uint64_t n_ops
         return fail(LLSV_ERR_ARG, "llsv_plan_passes: tile_bits must be in [1, %d]", kMaxTileBits);
     if (n_ops && !ops) return fail(LLSV_ERR_ARG, "llsv_plan_passes: ops is null");
     const int b = std::min(n_qubits, tile_bits);
-    const int c = plan_order(n_qubits, ops, n_ops, b, std::min(kLowBits, b), order, pass_end);
-    if (c < 0) return c;
-    *n_passes = uint64_t(c);
+    std::vector<Checked> info(n_ops);
+    for (uint64_t i = 0; i < n_ops; ++i)
+        if (int rc = check_op(n_qubits, ops[i], i, &info[i])) return rc;
+    // the schedule llsv_apply_program makes: register tiles when the tile has 12 bits
+    const std::shared_ptr<const PlanEntry> pe = cached_plan(n_qubits, b, b == kRegTileBits, ops, info);
+    uint64_t k = 0;
+    for (size_t p = 0; p < pe->passes.size(); ++p) {
+        for (uint64_t i : pe->passes[p].ops) {
+            if (order) order[k] = i;
+            ++k;
+        }
+        if (pass_end) pass_end[p] = k;
+    }
+    *n_passes = uint64_t(pe->passes.size());
     return LLSV_OK;
 }
