@@ -462,6 +462,7 @@ struct TiledPlan {
     int ncoef = 0;
     std::vector<int> coef_gate;  // gate index of coefficient j
     std::vector<TiledPass> fwd, bwd;
+    std::vector<int> slot_of;  // gradient slot of each rotation (backward-sweep order), else -1
 };
 
 // Schedule of the gate list as register-tile passes: ops in execution order; a one-qubit gate
@@ -470,7 +471,7 @@ struct TiledPlan {
 // order / pass_end: the schedule (planned here when empty; the backward sweep gets the forward
 // schedule reversed, which is valid for the reversed gate list and has as few passes).
 std::vector<TiledPass> tiled_passes(int n, const std::vector<const llsv_pgate*>& seq, const std::vector<int>& coef_of,
-                                    const std::vector<int>& slot_of, const std::vector<int>& gate_index, int ncoef,
+                                    std::vector<int>* slot_of, const std::vector<int>& gate_index, int ncoef,
                                     bool backward, std::vector<uint64_t>* order_io, std::vector<uint64_t>* end_io) {
     const int G = int(seq.size());
     std::vector<llsv_op> ops(G);
@@ -500,6 +501,8 @@ std::vector<TiledPass> tiled_passes(int n, const std::vector<const llsv_pgate*>&
         pass_end.resize(size_t(std::max(np, 0)));
     }
     const int np = int(pass_end.size());
+    int next_slot = 0;
+    for (int v : *slot_of) next_slot += v >= 0 ? 1 : 0;
     std::vector<TiledPass> out;
     uint64_t first = 0;
     for (int p = 0; p < np; ++p) {
@@ -551,7 +554,11 @@ std::vector<TiledPass> tiled_passes(int n, const std::vector<const llsv_pgate*>&
                     gr.kind = P_GRAD;
                     gr.a = local_of[g.q0];
                     gr.b = g.kind == LLSV_PG_RX ? 0 : g.kind == LLSV_PG_RY ? 1 : 2;
-                    gr.moff = slot_of[gi];
+                    // slots are numbered in the order the backward sweep meets them, so the
+                    // slots of one pass are a contiguous range
+                    int& sl = (*slot_of)[gi];
+                    if (sl < 0) sl = next_slot++;
+                    gr.moff = sl;
                     pops.push_back(gr);
                 }
                 po.kind = P_1QT;
@@ -578,7 +585,7 @@ std::map<std::string, std::unique_ptr<TiledPlan>>& tiled_cache() {
 
 // The tiled plan of a circuit (built and cached on first sight), or nullptr when the sweeps run
 // gate by gate: state below one tile, pass compiler off, or kernels not compiled (yet).
-TiledPlan* tiled_plan(int n, const llsv_pgate* gates, int G, const std::vector<int>& slot_of) {
+TiledPlan* tiled_plan(int n, const llsv_pgate* gates, int G) {
     if (n < kRegTileBits || G == 0 || jit_mode() == kJitOff) return nullptr;
     if (!(jit_mode() == kJitAlways || n >= kJitMinQubits)) return nullptr;
     std::string key = std::to_string(n) + ":";
@@ -604,7 +611,8 @@ TiledPlan* tiled_plan(int n, const llsv_pgate* gates, int G, const std::vector<i
                 idx[k] = k;
             }
             std::vector<uint64_t> forder, fend;
-            plan->fwd = tiled_passes(n, seq, coef_of, slot_of, idx, plan->ncoef, false, &forder, &fend);
+            plan->slot_of.assign(size_t(G), -1);
+            plan->fwd = tiled_passes(n, seq, coef_of, &plan->slot_of, idx, plan->ncoef, false, &forder, &fend);
             // the backward sweep: the forward passes last to first, each one's gates last to first
             // (positions in the reversed gate list)
             std::vector<uint64_t> border, bend;
@@ -615,7 +623,7 @@ TiledPlan* tiled_plan(int n, const llsv_pgate* gates, int G, const std::vector<i
             }
             std::reverse(seq.begin(), seq.end());
             std::reverse(idx.begin(), idx.end());
-            plan->bwd = tiled_passes(n, seq, coef_of, slot_of, idx, plan->ncoef, true, &border, &bend);
+            plan->bwd = tiled_passes(n, seq, coef_of, &plan->slot_of, idx, plan->ncoef, true, &border, &bend);
             it = tiled_cache().emplace(key, std::move(plan)).first;
         }
         tp = it->second.get();
@@ -671,12 +679,18 @@ int run(int n, int B, const llsv_pgate* gates, int G, int P, const double* param
     std::vector<PauliDev> ht(T);
     for (int t = 0; t < T; ++t) ht[t] = PauliDev{terms[t].coeff, terms[t].xmask, terms[t].zmask};
     CK(hipMemcpyAsync(dterms, ht.data(), sizeof(PauliDev) * T, hipMemcpyHostToDevice, st));
+    TiledPlan* tiled = tiled_plan(n, gates, G);
+    if (tiled) {
+        // the compiled sweeps number the slots themselves
+        slot_of = tiled->slot_of;
+        for (int k = 0; k < G; ++k)
+            if (slot_of[k] >= 0) slot_param[size_t(slot_of[k])] = gates[k].param;
+    }
     if (adj && !slot_param.empty())
         CK(hipMemcpyAsync(dslotmap, slot_param.data(), sizeof(int) * slot_param.size(), hipMemcpyHostToDevice, st));
 
     off += L.slotmap;
     double* table = reinterpret_cast<double*>(base + off);
-    TiledPlan* tiled = tiled_plan(n, gates, G, slot_of);
     const uint64_t tstride = 16 * uint64_t(tiled ? tiled->ncoef : 0);
     const int ntiles = int(tiled_tiles(n));
     if (tiled && tiled->ncoef) {
@@ -775,19 +789,16 @@ int llsv_pc_energy(int n, int batch, const llsv_pgate* gates, int n_gates, int n
 int llsv_pc_precompile(int n, const llsv_pgate* gates, int n_gates, int* forward_passes) {
     using namespace llsv;
     if (n < 1 || n > 30 || n_gates < 0 || (n_gates && !gates)) return fail(LLSV_ERR_ARG, "llsv_pc_precompile: bad arguments");
-    std::vector<int> slot_of(size_t(n_gates), -1);
-    int ns = 0;
     for (int k = 0; k < n_gates; ++k) {
         if (gates[k].kind < LLSV_PG_RX || gates[k].kind > LLSV_PG_CZ || gates[k].q0 < 0 || gates[k].q0 >= n ||
             (gates[k].kind >= LLSV_PG_CNOT && (gates[k].q1 < 0 || gates[k].q1 >= n || gates[k].q1 == gates[k].q0)))
             return fail(LLSV_ERR_ARG, "llsv_pc_precompile: gate %d: bad kind or qubits", k);
-        if (gates[k].kind <= LLSV_PG_RZ) slot_of[k] = ns++;
     }
     if (forward_passes) *forward_passes = 0;
     const int prev = jit_mode();
     if (prev == kJitOff || (prev == kJitAuto && n < kJitMinQubits)) return 0;
     set_jit_mode(kJitAlways);  // compile now, in this thread
-    TiledPlan* tp = tiled_plan(n, gates, n_gates, slot_of);
+    TiledPlan* tp = tiled_plan(n, gates, n_gates);
     set_jit_mode(prev);
     if (!tp) return 0;
     if (forward_passes) *forward_passes = int(tp->fwd.size());

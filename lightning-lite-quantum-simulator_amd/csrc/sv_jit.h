@@ -49,6 +49,10 @@ struct JitEntry;
 
 // The cache entry for a kernel source (created on first sight, not compiled).
 JitEntry* jit_entry(const std::string& body);
+
+// jit_entry(jit_source(ops, nops)), memoized on the ops' structure: a pass seen before (the same
+// circuit applied again, whatever its angles) finds its kernel without generating the source.
+JitEntry* jit_program_entry(const RegOp* ops, int nops);
 bool jit_ready(const JitEntry* e);
 
 // Asks for the entries to be compiled, all in one hipRTC program.  now: in the calling thread,

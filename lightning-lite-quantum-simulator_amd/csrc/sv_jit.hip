@@ -197,6 +197,14 @@ bool jit_wanted(int n, int nops, bool dense) {
 
 namespace {
 
+// The arithmetic form of an RO_1Q matrix: 2 for the RY pattern [[p, -q], [q, p]] (p, q real), 1 for
+// the RX pattern [[p, iq], [iq, p]], 0 for a general 2x2.
+int rotation_form(const double* m) {
+    if (m[1] == 0.0 && m[3] == 0.0 && m[5] == 0.0 && m[7] == 0.0 && m[0] == m[6] && m[2] == -m[4]) return 2;
+    if (m[1] == 0.0 && m[2] == 0.0 && m[4] == 0.0 && m[7] == 0.0 && m[0] == m[6] && m[3] == m[5]) return 1;
+    return 0;
+}
+
 // What a generated kernel looks like around the op sequence.
 struct Shape {
     bool batched = false;  // blockIdx.y = batch element: states A + b * size, coefficients T + b * tstride
@@ -267,14 +275,7 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
                 // RO_1QT carries its form; an RO_1Q matrix of the RX or RY pattern (zeros and equal
                 // entries where the rotation has them) runs the 8-operation form, which rounds exactly
                 // like the general update on such a matrix (the dropped terms are products with zero)
-                int form = op.kind == RO_1QT ? op.q[0] : 0;
-                if (op.kind == RO_1Q) {
-                    const double* m = op.m;
-                    if (m[1] == 0.0 && m[3] == 0.0 && m[5] == 0.0 && m[7] == 0.0 && m[0] == m[6] && m[2] == -m[4])
-                        form = 2;
-                    else if (m[1] == 0.0 && m[2] == 0.0 && m[4] == 0.0 && m[7] == 0.0 && m[0] == m[6] && m[3] == m[5])
-                        form = 1;
-                }
+                const int form = op.kind == RO_1QT ? op.q[0] : rotation_form(op.m);
                 const size_t t8 = size_t(8) * size_t(op.moff);
                 if (op.kind == RO_1Q)
                     o.line("{ const double* m = M + %zu;", c);
@@ -324,25 +325,39 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
             }
             case RO_GRAD: {
                 // Im <lambda| G |psi> over this lane's pairs along register bit a (G = X, Y or Z on it)
+                // Every product goes into a chain of FMAs (four per amplitude pair), alternating
+                // between two accumulators so that consecutive FMAs do not wait on each other.
                 const int K = 1 << op.a;
-                o.line("{ double g = 0.0;");
+                o.line("{ double g0 = 0.0, g1 = 0.0;");
+                auto fma4 = [&](int acc, int p, char cp, int q, char cq, bool neg0, int r, char cr, int s, char cs,
+                                bool neg1, int t, char ct, int v, char cv, bool neg2, int w, char cw, int x, char cx,
+                                bool neg3) {
+                    const char* g = acc ? "g1" : "g0";
+                    o.line("%s = __builtin_fma(%sl[%d].%c, a[%d].%c, %s); %s = __builtin_fma(%sl[%d].%c, a[%d].%c, %s); "
+                           "%s = __builtin_fma(%sl[%d].%c, a[%d].%c, %s); %s = __builtin_fma(%sl[%d].%c, a[%d].%c, %s);",
+                           g, neg0 ? "-" : "", p, cp, q, cq, g, g, neg1 ? "-" : "", r, cr, s, cs, g, g, neg2 ? "-" : "", t,
+                           ct, v, cv, g, g, neg3 ? "-" : "", w, cw, x, cx, g);
+                };
                 if (op.q[0] == 2 && op.tmask) {
-                    // Z on a lane bit: the sign is the lane's
-                    for (int j = 0; j < 16; ++j) o.line("g += l[%d].x * a[%d].y - l[%d].y * a[%d].x;", j, j, j, j);
+                    // Z on a lane bit: Im conj(l) a over all 16, the sign is the lane's
+                    for (int j = 0; j < 16; j += 2)
+                        fma4((j >> 1) & 1, j, 'x', j, 'y', false, j, 'y', j, 'x', true, j + 1, 'x', j + 1, 'y', false, j + 1,
+                             'y', j + 1, 'x', true);
+                    o.line("double g = g0 + g1;");
                     o.line("if ((lt & %uu) != 0u) g = -g;", op.tmask);
-                }
-                for (int j = 0; j < 16 && !op.tmask; ++j) {
-                    if (j & K) continue;
-                    const int u = j | K;
-                    if (op.q[0] == 2)
-                        o.line("g += (l[%d].x * a[%d].y - l[%d].y * a[%d].x) - (l[%d].x * a[%d].y - l[%d].y * a[%d].x);",
-                               j, j, j, j, u, u, u, u);
-                    else if (op.q[0] == 0)
-                        o.line("g += (l[%d].x * a[%d].y - l[%d].y * a[%d].x) + (l[%d].x * a[%d].y - l[%d].y * a[%d].x);",
-                               j, u, j, u, u, j, u, j);
-                    else
-                        o.line("g += (l[%d].x * a[%d].x + l[%d].y * a[%d].y) - (l[%d].x * a[%d].x + l[%d].y * a[%d].y);",
-                               u, j, u, j, j, u, j, u);
+                } else {
+                    int pair = 0;
+                    for (int j = 0; j < 16; ++j) {
+                        if (j & K) continue;
+                        const int u = j | K, acc = pair++ & 1;
+                        if (op.q[0] == 2)  // Im conj(l_j) a_j - Im conj(l_u) a_u
+                            fma4(acc, j, 'x', j, 'y', false, j, 'y', j, 'x', true, u, 'x', u, 'y', true, u, 'y', u, 'x', false);
+                        else if (op.q[0] == 0)  // Im conj(l_j) a_u + Im conj(l_u) a_j
+                            fma4(acc, j, 'x', u, 'y', false, j, 'y', u, 'x', true, u, 'x', j, 'y', false, u, 'y', j, 'x', true);
+                        else  // Re conj(l_u) a_j - Re conj(l_j) a_u
+                            fma4(acc, u, 'x', j, 'x', false, u, 'y', j, 'y', false, j, 'x', u, 'x', true, j, 'y', u, 'y', true);
+                    }
+                    o.line("double g = g0 + g1;");
                 }
                 o.line("#pragma unroll");
                 o.line("for (int o = 32; o > 0; o >>= 1) g += __shfl_xor(g, o, 64);");
@@ -452,6 +467,10 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
     int rq[4], t = 0;
     for (int bit = 0; bit < kRegTileBits; ++bit)
         if ((S >> bit) & 1u) rq[t++] = bit;
+    // When the pass ends in its starting layout, the store addresses equal the load addresses and
+    // the compiler would keep all of them (64-bit, one per register) live through the pass; the
+    // empty asm makes it compute them again here.
+    o.line("__asm__ volatile(\"\" : \"+v\"(lt));");
     o.line("{ unsigned long long gt = 0;");
     o.line("#pragma unroll");
     o.line("for (int j = 0; j < 12; ++j) if ((lt >> j) & 1u) gt |= lg[j];");
@@ -466,13 +485,24 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
            rq[0], rq[1], rq[2], rq[3]);
     o.line("__builtin_nontemporal_store(a[j], psi + o);%s } }", sh.two ? " __builtin_nontemporal_store(l[j], lam + o);" : "");
     if (!gslot.empty()) {
-        std::string list;
-        for (size_t k = 0; k < gslot.size(); ++k) list += (k ? ", " : "") + std::to_string(gslot[k]);
-        o.line("{ const int gslot[%zu] = {%s};", gslot.size(), list.c_str());
-        o.line("__syncthreads();");
-        o.line("for (unsigned int k = tid; k < %zuu; k += 256u)", gslot.size());
-        o.line("partials[((unsigned long long)blockIdx.y * nslot + gslot[k]) * ntiles + blockIdx.x] = "
-               "(gs[k][0] + gs[k][1]) + (gs[k][2] + gs[k][3]); }");
+        bool contiguous = true;
+        for (size_t k = 0; k < gslot.size(); ++k) contiguous = contiguous && gslot[k] == gslot[0] + int(k);
+        if (contiguous) {
+            // the usual case (slots numbered in sweep order): no table, which a thread would index
+            // at run time and the compiler would keep in scratch memory
+            o.line("__syncthreads();");
+            o.line("for (unsigned int k = tid; k < %zuu; k += 256u)", gslot.size());
+            o.line("partials[((unsigned long long)blockIdx.y * nslot + (%du + k)) * ntiles + blockIdx.x] = "
+                   "(gs[k][0] + gs[k][1]) + (gs[k][2] + gs[k][3]);", gslot[0]);
+        } else {
+            std::string list;
+            for (size_t k = 0; k < gslot.size(); ++k) list += (k ? ", " : "") + std::to_string(gslot[k]);
+            o.line("{ const int gslot[%zu] = {%s};", gslot.size(), list.c_str());
+            o.line("__syncthreads();");
+            o.line("for (unsigned int k = tid; k < %zuu; k += 256u)", gslot.size());
+            o.line("partials[((unsigned long long)blockIdx.y * nslot + gslot[k]) * ntiles + blockIdx.x] = "
+                   "(gs[k][0] + gs[k][1]) + (gs[k][2] + gs[k][3]); }");
+        }
     }
     o.line("}");
     return o.s;
@@ -481,6 +511,43 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
 }  // namespace
 
 std::string jit_source(const RegOp* ops, int nops) { return emit(ops, nops, Shape{}); }
+
+namespace {
+std::mutex g_sig_mu;
+std::unordered_map<std::string, JitEntry*>& sig_cache() {
+    static auto* c = new std::unordered_map<std::string, JitEntry*>();  // lives to process end
+    return *c;
+}
+}  // namespace
+
+JitEntry* jit_program_entry(const RegOp* ops, int nops) {
+    // everything the generated source depends on: per op its kind, bits, masks, coefficient slot
+    // and (RO_1Q) arithmetic form; the coefficients themselves are run-time data
+    std::string sig;
+    sig.reserve(size_t(nops) * 44);
+    auto put = [&](const void* p, size_t n) { sig.append(static_cast<const char*>(p), n); };
+    for (int i = 0; i < nops; ++i) {
+        const RegOp& op = ops[i];
+        const int form = op.kind == RO_1Q ? rotation_form(op.m) : -1;
+        put(&op.kind, sizeof op.kind);
+        put(&op.a, sizeof op.a);
+        put(&op.rmask, sizeof op.rmask);
+        put(&op.tmask, sizeof op.tmask);
+        put(&op.omask, sizeof op.omask);
+        put(&op.moff, sizeof op.moff);
+        put(op.q, sizeof op.q);
+        put(&form, sizeof form);
+    }
+    {
+        std::lock_guard<std::mutex> lock(g_sig_mu);
+        auto it = sig_cache().find(sig);
+        if (it != sig_cache().end()) return it->second;
+    }
+    JitEntry* e = jit_entry(jit_source(ops, nops));
+    std::lock_guard<std::mutex> lock(g_sig_mu);
+    sig_cache().emplace(std::move(sig), e);
+    return e;
+}
 
 std::string jit_adjoint_source(const RegOp* ops, int nops, bool backward) {
     Shape sh;

@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <complex>
 #include <cstring>
 #include <list>
 #include <memory>
@@ -691,6 +693,196 @@ void merge_1q(PassOp* prev, const PassOp& next) {
     prev->kind = (prev->kind == P_DIAG && next.kind == P_DIAG) ? P_DIAG : P_1Q;
 }
 
+// ---- one-qubit unitaries split around their phases -----------------------------------------
+// A merged one-qubit unitary U costs 16 FP64 operations per amplitude pair as a general 2x2.
+// Written U = diag(p0, p1) R diag(1, g) with R = [[c, -s], [s, c]] real, it costs 8 (R) plus 4
+// for g on the half of the registers that hold bit 1 -- and diag(p0, p1) need not be applied at
+// all: it commutes with every diagonal op (CZ, MCZ, phases) and with a CNOT it controls, so it is
+// carried forward on its qubit, into the next unitary on that qubit (in this pass or a later
+// one), as a global phase p0 and a relative phase p1 / p0.  What is left at the end of the program
+// becomes per-tile scalars (qubits outside the last tile) or diagonal ops.  Both register-tile
+// kernels (interpreted and compiled) see the rewritten ops, so they still agree bit for bit.
+using cx = std::complex<double>;
+
+// U (row-major interleaved) = diag(p0, p1) [[c, -s], [s, c]] diag(1, g), when U is unitary.
+bool split_unitary(const double* m, double* c, double* s, cx* p0, cx* p1, cx* g) {
+    const cx a(m[0], m[1]), b(m[2], m[3]), u10(m[4], m[5]), d(m[6], m[7]);
+    const double tol = 1e-12;
+    if (std::abs(std::norm(a) + std::norm(u10) - 1.0) > tol || std::abs(std::norm(b) + std::norm(d) - 1.0) > tol ||
+        std::abs(std::conj(a) * b + std::conj(u10) * d) > tol)
+        return false;
+    const double C = std::abs(a), S = std::abs(u10), h = std::hypot(C, S);
+    cx P0, P1, G;
+    if (C >= S) {
+        P0 = a / C;
+        if (S > 0.0) {
+            P1 = u10 / S;
+            G = d / (P1 * C);
+        } else {
+            G = 1.0;
+            P1 = d / C;
+        }
+    } else {
+        P1 = u10 / S;
+        if (C > 0.0) {
+            P0 = a / C;
+            G = -b / (P0 * S);
+        } else {
+            G = 1.0;
+            P0 = -b / S;
+        }
+    }
+    P0 /= std::abs(P0);
+    P1 /= std::abs(P1);
+    G /= std::abs(G);
+    *c = C / h;
+    *s = S / h;
+    // the factors must give U back (a matrix that only passes the unitarity test loosely may not)
+    const cx r[4] = {P0 * *c, -P0 * *s * G, P1 * *s, P1 * *c * G};
+    const cx u[4] = {a, b, u10, d};
+    for (int i = 0; i < 4; ++i)
+        if (std::abs(r[i] - u[i]) > 1e-13) return false;
+    *p0 = P0;
+    *p1 = P1;
+    *g = G;
+    return true;
+}
+
+bool unit_modulus(cx z) { return std::abs(std::norm(z) - 1.0) <= 1e-12; }
+
+// Phases carried between the passes of one program: a global factor and, per qubit, the relative
+// phase of its |1> (the diagonal diag(1, rel[q])).
+struct CarriedPhases {
+    cx glob = 1.0;
+    std::vector<cx> rel;
+};
+
+PassOp diag_op(int bit, cx d0, cx d1, bool resident) {
+    PassOp po{};
+    po.kind = P_DIAG;
+    po.a = bit;
+    po.b = resident ? 1 : 0;
+    po.m[0] = d0.real();
+    po.m[1] = d0.imag();
+    po.m[6] = d1.real();
+    po.m[7] = d1.imag();
+    return po;
+}
+
+PassOp scale_op(int qubit, cx d0, cx d1) {
+    PassOp po = diag_op(0, d0, d1, false);
+    po.kind = P_SCALE;
+    po.a = qubit;
+    return po;
+}
+
+// Rewrites a register-tile pass's ops (after merging): unitaries split as above where the phase
+// they leave can move on, diagonal unit-modulus ops carried.  tq: global qubit of each tile bit.
+// last: the program's final pass, which applies whatever is still carried.
+void carry_phases(std::vector<PassOp>& pops, const int* tq, int b, int n, bool last, CarriedPhases* cp) {
+    std::vector<PassOp> out;
+    out.reserve(pops.size() + 8);
+    auto flush = [&](int bit, bool resident) {
+        cx& r = cp->rel[size_t(tq[bit])];
+        if (r != 1.0) out.push_back(diag_op(bit, 1.0, r, resident));
+        r = 1.0;
+    };
+    for (size_t i = 0; i < pops.size(); ++i) {
+        PassOp po = pops[i];
+        switch (po.kind) {
+            case P_1Q: {
+                const int t = po.a;
+                cx& r = cp->rel[size_t(tq[t])];
+                // U diag(1, r): the carried phase goes into U's second column
+                const cx u01 = cx(po.m[2], po.m[3]) * r, u11 = cx(po.m[6], po.m[7]) * r;
+                po.m[2] = u01.real();
+                po.m[3] = u01.imag();
+                po.m[6] = u11.real();
+                po.m[7] = u11.imag();
+                r = 1.0;
+                // split only when the phase left over can travel: the next op on this bit commutes
+                // with a diagonal (or there is none in this pass)
+                bool movable = true;
+                for (size_t k = i + 1; k < pops.size(); ++k) {
+                    const PassOp& nx = pops[k];
+                    if (nx.kind == P_CNOT && (nx.a == t || nx.b == t)) {
+                        movable = nx.a == t;
+                        break;
+                    }
+                    if (nx.kind == P_DENSE) {
+                        bool hit = false;
+                        for (int q = 0; q < nx.a; ++q) hit = hit || nx.q[q] == t;
+                        if (hit) {
+                            movable = false;
+                            break;
+                        }
+                    }
+                    if ((nx.kind == P_1Q || nx.kind == P_DIAG) && nx.a == t) break;
+                }
+                double c, s;
+                cx p0, p1, g;
+                if (!movable || !split_unitary(po.m, &c, &s, &p0, &p1, &g)) {
+                    out.push_back(po);
+                    break;
+                }
+                if (g != 1.0) out.push_back(diag_op(t, 1.0, g, true));
+                PassOp rot{};
+                rot.kind = P_1Q;
+                rot.a = t;
+                rot.m[0] = c;
+                rot.m[2] = -s;
+                rot.m[4] = s;
+                rot.m[6] = c;
+                out.push_back(rot);
+                cp->glob *= p0;
+                r = p1 / p0;
+                break;
+            }
+            case P_DIAG:
+            case P_SCALE: {
+                const cx d0(po.m[0], po.m[1]), d1(po.m[6], po.m[7]);
+                if (!unit_modulus(d0) || !unit_modulus(d1)) {
+                    out.push_back(po);
+                    break;
+                }
+                const int q = po.kind == P_DIAG ? tq[po.a] : po.a;
+                cp->glob *= d0;
+                cp->rel[size_t(q)] *= d1 / d0;
+                break;
+            }
+            case P_CNOT:
+                flush(po.b, true);  // the target's phase does not commute with it
+                out.push_back(po);
+                break;
+            case P_DENSE:
+                for (int q = 0; q < po.a; ++q) flush(po.q[q], false);
+                out.push_back(po);
+                break;
+            default:  // P_MCZ: diagonal
+                out.push_back(po);
+                break;
+        }
+    }
+    if (last) {
+        std::vector<char> in_tile(size_t(n), 0);
+        for (int t = 0; t < b; ++t) in_tile[size_t(tq[t])] = 1;
+        int outside = -1;
+        for (int q = 0; q < n; ++q) {
+            if (!in_tile[size_t(q)]) outside = q;
+            cx& r = cp->rel[size_t(q)];
+            if (r == 1.0 || in_tile[size_t(q)]) continue;
+            out.push_back(scale_op(q, 1.0, r));
+            r = 1.0;
+        }
+        for (int t = 0; t < b; ++t) flush(t, false);
+        if (cp->glob != 1.0) {
+            out.push_back(outside >= 0 ? scale_op(outside, cp->glob, cp->glob) : diag_op(0, cp->glob, cp->glob, false));
+            cp->glob = 1.0;
+        }
+    }
+    pops.swap(out);
+}
+
 // Local index bits of register j when the register bits are S (4 bits of a 12-bit mask):
 // bit t of j lands on the t-th lowest bit of S.
 uint32_t reg_local(uint32_t S, int j) {
@@ -730,6 +922,7 @@ std::vector<RegOp> to_register_program(const std::vector<PassOp>& in, int L, int
     constexpr uint32_t kStart = 0xF00u;  // local bits 8..11
     const uint32_t low = (1u << L) - 1;
     auto need_of = [](const PassOp& po) -> uint32_t {
+        if (po.kind == P_DIAG && po.b) return 1u << po.a;  // wants its bit in registers (half the work)
         if (po.kind == P_1QT && po.b == 3) return 0u;   // diagonal: any residency
         if (po.kind == P_GRAD && po.b == 2) return 0u;  // Z generator: any residency
         if (po.kind == P_1Q || po.kind == P_1QT || po.kind == P_GRAD) return 1u << po.a;
@@ -893,6 +1086,12 @@ int build_program(int n, const llsv_op* ops, uint64_t n_ops, int tile_bits, Buil
     const int L = pe->L;
     const std::vector<PlannedPass>& passes = pe->passes;
     out->launches.reserve(passes.size());
+    // phases are carried between passes only when every pass runs in register tiles
+    bool carry = reg_tiles;
+    for (const PlannedPass& p : passes)
+        carry = carry && !(p.ops.size() == 1 && (info[p.ops[0]].single || ops[p.ops[0]].kind == LLSV_OP_DENSE) && n > b);
+    CarriedPhases cp;
+    cp.rel.assign(size_t(n), cx(1.0));
     for (const PlannedPass& p : passes) {
         Launch la{};
         if (p.ops.size() == 1 && info[p.ops[0]].single && n > b) {
@@ -1001,6 +1200,7 @@ int build_program(int n, const llsv_op* ops, uint64_t n_ops, int tile_bits, Buil
             }
             pops.push_back(po);
         }
+        if (carry) carry_phases(pops, d.tq, b, n, &p == &passes.back(), &cp);
         if (reg_tiles) {
             const std::vector<RegOp> rops = to_register_program(pops, L, d.rq);
             d.nops = int(rops.size());
@@ -1063,7 +1263,7 @@ int run_program(llsv_state* s, const llsv_op* ops, uint64_t n_ops) {
         for (size_t i = 0; i < bp.launches.size(); ++i) {
             const Launch& la = bp.launches[i];
             if (la.streamed || la.mfma || !jit_wanted(n, la.d.nops, la.dense)) continue;
-            jit[i] = jit_entry(jit_source(bp.rops.data() + la.first, la.d.nops));
+            jit[i] = jit_program_entry(bp.rops.data() + la.first, la.d.nops);
             if (!jit_ready(jit[i])) want.push_back(jit[i]);
         }
         if (!want.empty()) jit_request(want, jit_mode() == kJitAlways);
@@ -1137,7 +1337,7 @@ int jit_precompile(int n, const llsv_op* ops, uint64_t n_ops) {
     std::vector<JitEntry*> want;
     for (const Launch& la : bp.launches) {
         if (la.streamed || la.mfma || !jit_wanted(n, la.d.nops, la.dense)) continue;
-        want.push_back(jit_entry(jit_source(bp.rops.data() + la.first, la.d.nops)));
+        want.push_back(jit_program_entry(bp.rops.data() + la.first, la.d.nops));
     }
     jit_request(want, true);
     int count = 0;
