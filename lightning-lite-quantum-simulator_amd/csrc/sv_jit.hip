@@ -323,6 +323,21 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
                 o.line("}");
                 break;
             }
+            case RO_1QN: {
+                const int K = 1 << op.a;
+                o.line("{ const double* m = M + %zu; const double br = m[0], bi = m[1], cr = m[2], ci = m[3];", c);
+                for (int k = 0; k < na; ++k)
+                    for (int j = 0; j < 16; ++j)
+                        if (!(j & K)) o.line("r_nu%d(%s[%d], %s[%d], br, bi, cr, ci);", op.q[0], arrs[k], j, arrs[k], j | K);
+                o.line("}");
+                break;
+            }
+            case RO_RSCALE:
+                o.line("{ const double f = M[%zu];", c);
+                for (int k = 0; k < na; ++k)
+                    for (int j = 0; j < 16; ++j) o.line("%s[%d] *= f;", arrs[k], j);
+                o.line("}");
+                break;
             case RO_GRAD: {
                 // Im <lambda| G |psi> over this lane's pairs along register bit a (G = X, Y or Z on it)
                 // Every product goes into a chain of FMAs (four per amplitude pair), alternating
@@ -475,9 +490,12 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
     o.line("#pragma unroll");
     o.line("for (int j = 0; j < 12; ++j) if ((lt >> j) & 1u) gt |= lg[j];");
     if (scaled) {
-        o.line("if (sr != 1.0 || si != 0.0) {");
+        o.line("if (si != 0.0) {");
         o.line("#pragma unroll");
-        o.line("for (int j = 0; j < 16; ++j) { r_sc(a[j], sr, si);%s } }", sh.two ? " r_sc(l[j], sr, si);" : "");
+        o.line("for (int j = 0; j < 16; ++j) { r_sc(a[j], sr, si);%s }", sh.two ? " r_sc(l[j], sr, si);" : "");
+        o.line("} else if (sr != 1.0) {");
+        o.line("#pragma unroll");
+        o.line("for (int j = 0; j < 16; ++j) { a[j] *= sr;%s } }", sh.two ? " l[j] *= sr;" : "");
     }
     o.line("#pragma unroll");
     o.line("for (int j = 0; j < 16; ++j) { const unsigned long long o = base | gt | ((j & 1) ? lg[%d] : 0ull) | "
@@ -590,7 +608,23 @@ std::unordered_map<std::string, std::unique_ptr<JitEntry>>& cache() {
 // The process keeps its own cache of compiled passes; keep the code-object manager from writing
 // one to disk.  Set once when the library is loaded, before it starts any thread (setenv must not
 // race with getenv on another thread, and the compiler worker reads the environment).
-__attribute__((constructor)) void jit_env_init() { setenv("AMD_COMGR_CACHE", "0", 0); }
+// LLSV_JIT_DUMP=<directory>: every compile batch's source is also written there, as
+// <first kernel name>.hip, for reading the generated passes (and compiling them by hand).
+std::string* g_dump_dir = nullptr;
+__attribute__((constructor)) void jit_env_init() {
+    setenv("AMD_COMGR_CACHE", "0", 0);
+    const char* d = std::getenv("LLSV_JIT_DUMP");
+    if (d && *d) g_dump_dir = new std::string(d);
+}
+
+void dump_source(const std::string& name, const std::string& src) {
+    if (!g_dump_dir) return;
+    const std::string path = *g_dump_dir + "/" + name + ".hip";
+    if (FILE* f = std::fopen(path.c_str(), "w")) {
+        std::fwrite(src.data(), 1, src.size(), f);
+        std::fclose(f);
+    }
+}
 
 // Compiles the entries (state kBusy, set by the caller) as one hipRTC program, without the lock.
 void compile_batch(const std::vector<JitEntry*>& batch) {
@@ -605,6 +639,7 @@ void compile_batch(const std::vector<JitEntry*>& batch) {
         body.replace(body.find("LLSV_KERNEL"), 11, names[i]);
         src += body;
     }
+    dump_source(names.front(), src);
     auto obj = std::make_shared<CodeObject>();
     std::string error;
     hiprtcProgram prog = nullptr;

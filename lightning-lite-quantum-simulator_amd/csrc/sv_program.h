@@ -36,6 +36,11 @@ enum PassOpKind : int {
                    // 3 diagonal (RZ; needs no register residency)
     P_GRAD = 7,    // a = local bit, b = generator (0 X, 1 Y, 2 Z), moff = gradient slot:
                    // accumulate Im <lambda| G |psi> over the tile (Z needs no register residency)
+    // register-resident passes with carried phases only:
+    P_1QN = 8,     // a = local bit, a one-qubit matrix without its row scales (r_nu0 / r_nu1):
+                   // m[0..3] = b, c; form b = 0 for (x + b y, c x + y), 1 for (b x + y, x + c y)
+    P_RSCALE = 9,  // every amplitude of the tile times the real m[0] (row scales taken out of
+                   // P_1QN ops before it)
 };
 
 struct PassOp {
@@ -74,6 +79,8 @@ enum RegOpKind : int {
                     // a lane bit (compiled passes only)
     RO_GRAD = 8,    // a = register bit, q[0] = generator (0 X, 1 Y, 2 Z), moff = gradient slot;
                     // Z on a lane bit: tmask = that bit
+    RO_1QN = 9,     // a = register bit of the target, q[0] = form, m as P_1QN
+    RO_RSCALE = 10, // every register times the real m[0]
 };
 
 struct RegOp {

@@ -237,6 +237,18 @@ __device__ __forceinline__ void reg_cswap(v2d* a, bool tok) {
         }
 }
 
+// RO_1QN on register bit K in form F.
+template <int K, int F>
+__device__ __forceinline__ void reg_1qn(v2d* a, const double* m) {
+    const double br = m[0], bi = m[1], cr = m[2], ci = m[3];
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+        if (!(j & (1 << K))) {
+            if (F) r_nu1(a[j], a[j | (1 << K)], br, bi, cr, ci);
+            else r_nu0(a[j], a[j | (1 << K)], br, bi, cr, ci);
+        }
+}
+
 template <bool DENSE>
 __global__ __launch_bounds__(256) void k_pass_reg(v2d* __restrict__ psi, PassDesc d, const RegOp* __restrict__ ops,
                                                   const v2d* __restrict__ mats) {
@@ -277,6 +289,22 @@ __global__ __launch_bounds__(256) void k_pass_reg(v2d* __restrict__ psi, PassDes
             if (k == 1) reg_1q<1>(a, op.m);
             if (k == 2) reg_1q<2>(a, op.m);
             if (k == 3) reg_1q<3>(a, op.m);
+        }
+        if (kind == RO_1QN) {
+            const int k = op.a + 4 * op.q[0];
+            if (k == 0) reg_1qn<0, 0>(a, op.m);
+            if (k == 1) reg_1qn<1, 0>(a, op.m);
+            if (k == 2) reg_1qn<2, 0>(a, op.m);
+            if (k == 3) reg_1qn<3, 0>(a, op.m);
+            if (k == 4) reg_1qn<0, 1>(a, op.m);
+            if (k == 5) reg_1qn<1, 1>(a, op.m);
+            if (k == 6) reg_1qn<2, 1>(a, op.m);
+            if (k == 7) reg_1qn<3, 1>(a, op.m);
+        }
+        if (kind == RO_RSCALE) {
+            const double f = op.m[0];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) a[j] *= f;
         }
         if (kind == RO_MCZ && (base & op.omask) == op.omask) {
             const uint32_t sgn = ((lt & op.tmask) == op.tmask) ? 0x80000000u : 0u;
@@ -360,9 +388,12 @@ __global__ __launch_bounds__(256) void k_pass_reg(v2d* __restrict__ psi, PassDes
     uint64_t gs[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) gs[t] = local_to_global(d, d.rq[t]);
-    if (sr != 1.0 || si != 0.0) {
+    if (si != 0.0) {
 #pragma unroll
         for (int j = 0; j < 16; ++j) r_sc(a[j], sr, si);
+    } else if (sr != 1.0) {  // a real factor: two products per amplitude
+#pragma unroll
+        for (int j = 0; j < 16; ++j) a[j] *= sr;
     }
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
@@ -693,60 +724,19 @@ void merge_1q(PassOp* prev, const PassOp& next) {
     prev->kind = (prev->kind == P_DIAG && next.kind == P_DIAG) ? P_DIAG : P_1Q;
 }
 
-// ---- one-qubit unitaries split around their phases -----------------------------------------
-// A merged one-qubit unitary U costs 16 FP64 operations per amplitude pair as a general 2x2.
-// Written U = diag(p0, p1) R diag(1, g) with R = [[c, -s], [s, c]] real, it costs 8 (R) plus 4
-// for g on the half of the registers that hold bit 1 -- and diag(p0, p1) need not be applied at
-// all: it commutes with every diagonal op (CZ, MCZ, phases) and with a CNOT it controls, so it is
-// carried forward on its qubit, into the next unitary on that qubit (in this pass or a later
-// one), as a global phase p0 and a relative phase p1 / p0.  What is left at the end of the program
-// becomes per-tile scalars (qubits outside the last tile) or diagonal ops.  Both register-tile
-// kernels (interpreted and compiled) see the rewritten ops, so they still agree bit for bit.
+// ---- one-qubit matrices without their row scales ---------------------------------------------
+// A merged one-qubit matrix U = [[a, b], [c, d]] costs 16 FP64 operations per amplitude pair as a
+// general 2x2.  Written U = diag(a, d) [[1, b/a], [c/d, 1]] (or diag(b, c) [[a/b, 1], [1, d/c]]
+// when |b c| > |a d|), the right-hand factor costs 8, one FMA per product (r_nu0 / r_nu1) -- and
+// the diagonal need not be applied where it stands: its modulus |a| (|b|) is a real factor of the
+// whole tile, multiplied up over the pass and applied once at its end (P_RSCALE), and the rest,
+// diag(a, d) / |a|, commutes with every diagonal op (CZ, MCZ, phases) and with a CNOT it controls,
+// so it is carried forward on its qubit, into the next matrix on that qubit (in this pass or a
+// later one), as a global factor and a relative phase d / a.  What is left at the end of the
+// program becomes per-tile scalars (qubits outside the last tile) or diagonal ops; a CNOT target
+// or a dense block applies what its qubits carry first.  Both register-tile kernels (interpreted
+// and compiled) see the rewritten ops, so they still agree bit for bit.
 using cx = std::complex<double>;
-
-// U (row-major interleaved) = diag(p0, p1) [[c, -s], [s, c]] diag(1, g), when U is unitary.
-bool split_unitary(const double* m, double* c, double* s, cx* p0, cx* p1, cx* g) {
-    const cx a(m[0], m[1]), b(m[2], m[3]), u10(m[4], m[5]), d(m[6], m[7]);
-    const double tol = 1e-12;
-    if (std::abs(std::norm(a) + std::norm(u10) - 1.0) > tol || std::abs(std::norm(b) + std::norm(d) - 1.0) > tol ||
-        std::abs(std::conj(a) * b + std::conj(u10) * d) > tol)
-        return false;
-    const double C = std::abs(a), S = std::abs(u10), h = std::hypot(C, S);
-    cx P0, P1, G;
-    if (C >= S) {
-        P0 = a / C;
-        if (S > 0.0) {
-            P1 = u10 / S;
-            G = d / (P1 * C);
-        } else {
-            G = 1.0;
-            P1 = d / C;
-        }
-    } else {
-        P1 = u10 / S;
-        if (C > 0.0) {
-            P0 = a / C;
-            G = -b / (P0 * S);
-        } else {
-            G = 1.0;
-            P0 = -b / S;
-        }
-    }
-    P0 /= std::abs(P0);
-    P1 /= std::abs(P1);
-    G /= std::abs(G);
-    *c = C / h;
-    *s = S / h;
-    // the factors must give U back (a matrix that only passes the unitarity test loosely may not)
-    const cx r[4] = {P0 * *c, -P0 * *s * G, P1 * *s, P1 * *c * G};
-    const cx u[4] = {a, b, u10, d};
-    for (int i = 0; i < 4; ++i)
-        if (std::abs(r[i] - u[i]) > 1e-13) return false;
-    *p0 = P0;
-    *p1 = P1;
-    *g = G;
-    return true;
-}
 
 bool unit_modulus(cx z) { return std::abs(std::norm(z) - 1.0) <= 1e-12; }
 
@@ -782,6 +772,14 @@ PassOp scale_op(int qubit, cx d0, cx d1) {
 void carry_phases(std::vector<PassOp>& pops, const int* tq, int b, int n, bool last, CarriedPhases* cp) {
     std::vector<PassOp> out;
     out.reserve(pops.size() + 8);
+    double rscale = 1.0;  // product of the row scales taken out of P_1QN ops not yet applied
+    auto emit_rscale = [&] {
+        PassOp rs{};
+        rs.kind = P_RSCALE;
+        rs.m[0] = rscale;
+        out.push_back(rs);
+        rscale = 1.0;
+    };
     auto flush = [&](int bit, bool resident) {
         cx& r = cp->rel[size_t(tq[bit])];
         if (r != 1.0) out.push_back(diag_op(bit, 1.0, r, resident));
@@ -794,47 +792,44 @@ void carry_phases(std::vector<PassOp>& pops, const int* tq, int b, int n, bool l
                 const int t = po.a;
                 cx& r = cp->rel[size_t(tq[t])];
                 // U diag(1, r): the carried phase goes into U's second column
-                const cx u01 = cx(po.m[2], po.m[3]) * r, u11 = cx(po.m[6], po.m[7]) * r;
-                po.m[2] = u01.real();
-                po.m[3] = u01.imag();
-                po.m[6] = u11.real();
-                po.m[7] = u11.imag();
-                r = 1.0;
-                // split only when the phase left over can travel: the next op on this bit commutes
-                // with a diagonal (or there is none in this pass)
-                bool movable = true;
-                for (size_t k = i + 1; k < pops.size(); ++k) {
-                    const PassOp& nx = pops[k];
-                    if (nx.kind == P_CNOT && (nx.a == t || nx.b == t)) {
-                        movable = nx.a == t;
-                        break;
-                    }
-                    if (nx.kind == P_DENSE) {
-                        bool hit = false;
-                        for (int q = 0; q < nx.a; ++q) hit = hit || nx.q[q] == t;
-                        if (hit) {
-                            movable = false;
-                            break;
-                        }
-                    }
-                    if ((nx.kind == P_1Q || nx.kind == P_DIAG) && nx.a == t) break;
-                }
-                double c, s;
-                cx p0, p1, g;
-                if (!movable || !split_unitary(po.m, &c, &s, &p0, &p1, &g)) {
+                const cx a(po.m[0], po.m[1]), u10(po.m[4], po.m[5]);
+                const cx b = cx(po.m[2], po.m[3]) * r, d = cx(po.m[6], po.m[7]) * r;
+                // U = diag(a, d) [[1, b/a], [c/d, 1]] (form 0) or diag(b, c) [[a/b, 1], [1, d/c]]
+                // (form 1).  The divisions lose no accuracy (the quotient's rounding error, scaled
+                // back, is relative to the term it multiplies); the bound only keeps the quotients
+                // and the pass's scale in range.  Form 0 whenever its divisors are not below 1e-6
+                // of the other pair, so that angles drifting between calls almost never change a
+                // pass's kernel; the row factors leave as |a| (|b|) into the tile's real scale and a
+                // diagonal carried on the qubit like any other.
+                const double na = std::abs(a), nb = std::abs(b), nc = std::abs(u10), nd = std::abs(d);
+                const double kRatio = 1e-6;
+                int form = -1;
+                if (na > 0.0 && nd > 0.0 && std::min(na, nd) >= kRatio * std::max(nb, nc)) form = 0;
+                else if (nb > 0.0 && nc > 0.0 && std::min(nb, nc) >= kRatio * std::max(na, nd)) form = 1;
+                if (form < 0) {
+                    po.m[2] = b.real();
+                    po.m[3] = b.imag();
+                    po.m[6] = d.real();
+                    po.m[7] = d.imag();
+                    r = 1.0;
                     out.push_back(po);
                     break;
                 }
-                if (g != 1.0) out.push_back(diag_op(t, 1.0, g, true));
-                PassOp rot{};
-                rot.kind = P_1Q;
-                rot.a = t;
-                rot.m[0] = c;
-                rot.m[2] = -s;
-                rot.m[4] = s;
-                rot.m[6] = c;
-                out.push_back(rot);
-                cp->glob *= p0;
+                const cx p0 = form ? b : a, p1 = form ? u10 : d;
+                const cx beta = form ? a / b : b / a, gamma = form ? d / u10 : u10 / d;
+                const double sig = std::abs(p0);
+                PassOp nu{};
+                nu.kind = P_1QN;
+                nu.a = t;
+                nu.b = form;
+                nu.m[0] = beta.real();
+                nu.m[1] = beta.imag();
+                nu.m[2] = gamma.real();
+                nu.m[3] = gamma.imag();
+                out.push_back(nu);
+                rscale *= sig;
+                if (rscale < 1e-100 || rscale > 1e100) emit_rscale();
+                cp->glob *= p0 / sig;
                 r = p1 / p0;
                 break;
             }
@@ -863,6 +858,7 @@ void carry_phases(std::vector<PassOp>& pops, const int* tq, int b, int n, bool l
                 break;
         }
     }
+    if (rscale != 1.0) emit_rscale();
     if (last) {
         std::vector<char> in_tile(size_t(n), 0);
         for (int t = 0; t < b; ++t) in_tile[size_t(tq[t])] = 1;
@@ -925,7 +921,7 @@ std::vector<RegOp> to_register_program(const std::vector<PassOp>& in, int L, int
         if (po.kind == P_DIAG && po.b) return 1u << po.a;  // wants its bit in registers (half the work)
         if (po.kind == P_1QT && po.b == 3) return 0u;   // diagonal: any residency
         if (po.kind == P_GRAD && po.b == 2) return 0u;  // Z generator: any residency
-        if (po.kind == P_1Q || po.kind == P_1QT || po.kind == P_GRAD) return 1u << po.a;
+        if (po.kind == P_1Q || po.kind == P_1QN || po.kind == P_1QT || po.kind == P_GRAD) return 1u << po.a;
         if (po.kind == P_CNOT) return 1u << po.b;
         return 0u;
     };
@@ -957,6 +953,9 @@ std::vector<RegOp> to_register_program(const std::vector<PassOp>& in, int L, int
                 return uint32_t(po.lmask);
             case P_SCALE:
                 return 0u;  // a per-tile scalar commutes with every op of the tile
+            case P_RSCALE:
+                // keeps its place: it undoes the growth of the P_1QN ops before it (range only)
+                return (1u << kMaxTileBits) - 1u;
             case P_DENSE: {
                 uint32_t m = 0;
                 for (int t = 0; t < po.a; ++t) m |= 1u << po.q[t];
@@ -1031,6 +1030,14 @@ std::vector<RegOp> to_register_program(const std::vector<PassOp>& in, int L, int
             case P_1Q:
                 ro.kind = RO_1Q;
                 ro.a = rank(S, po.a);
+                break;
+            case P_1QN:
+                ro.kind = RO_1QN;
+                ro.a = rank(S, po.a);
+                ro.q[0] = po.b;
+                break;
+            case P_RSCALE:
+                ro.kind = RO_RSCALE;
                 break;
             case P_CNOT: {
                 ro.kind = RO_CNOT;

@@ -19,6 +19,9 @@
 //   the components go through scalar locals (a bit cast of a vector element expression such
 //   as v.y reads element 0 under this compiler).
 // r_phase: (sr + i si) <- (sr + i si)(dr + i di).
+// r_nu0 / r_nu1: a one-qubit matrix with its row scales taken out (normalised form), one FMA
+//   per product: r_nu0 (x, y) <- (x + b y, c x + y), r_nu1 (x, y) <- (b x + y, x + c y), with
+//   b = m[0] + i m[1], c = m[2] + i m[3].
 #define LLSV_REGOPS_SOURCE                                                                              \
     __device__ __forceinline__ unsigned int r_swz(unsigned int l) { return l ^ ((l >> 4) & 15u); }      \
     __device__ __forceinline__ unsigned int r_ins0(unsigned int x, int pos) {                          \
@@ -65,6 +68,20 @@
         const double nr = __builtin_fma(sr, dr, -si * di);                                              \
         si = __builtin_fma(sr, di, si * dr);                                                            \
         sr = nr;                                                                                        \
+    }                                                                                                   \
+    __device__ __forceinline__ void r_nu0(v2d& x, v2d& y, double br, double bi, double cr, double ci) { \
+        const double xr = x.x, xi = x.y, yr = y.x, yi = y.y;                                            \
+        x.x = __builtin_fma(-bi, yi, __builtin_fma(br, yr, xr));                                        \
+        x.y = __builtin_fma(bi, yr, __builtin_fma(br, yi, xi));                                         \
+        y.x = __builtin_fma(-ci, xi, __builtin_fma(cr, xr, yr));                                        \
+        y.y = __builtin_fma(ci, xr, __builtin_fma(cr, xi, yi));                                         \
+    }                                                                                                   \
+    __device__ __forceinline__ void r_nu1(v2d& x, v2d& y, double br, double bi, double cr, double ci) { \
+        const double xr = x.x, xi = x.y, yr = y.x, yi = y.y;                                            \
+        x.x = __builtin_fma(-bi, xi, __builtin_fma(br, xr, yr));                                        \
+        x.y = __builtin_fma(bi, xr, __builtin_fma(br, xi, yi));                                         \
+        y.x = __builtin_fma(-ci, yi, __builtin_fma(cr, yr, xr));                                        \
+        y.y = __builtin_fma(ci, yr, __builtin_fma(cr, yi, xi));                                         \
     }
 
 #define LLSV_STRINGIZE2(...) #__VA_ARGS__
