@@ -440,8 +440,24 @@ hipError_t apply_gate(hipStream_t st, v2d* A, v2d* Lm, bool two, const Layout& L
 }
 
 // ---------------------------------------------------------------- tiled sweeps
+// An RX / RY entry of the table holds, in place of its first two coefficients, the shears
+// (a, b) that the compiled passes apply (r_shx / r_shy): the rotation by f = theta / 2 is -1
+// times the rotation by f - pi, so f is folded into [-pi/2, pi/2] first, where
+// a = -tan(f/2) = -sin f / (1 + cos f) and b = +-sin f are at most 1 in modulus.  The dropped
+// sign multiplies a whole state; psi and lambda carry the same sign, so energies and gradients
+// do not see it.
+__device__ __forceinline__ void rotation_shears(const PG& g, double* m) {
+    double c = m[0], s = g.kind == LLSV_PG_RX ? -m[5] : m[4];
+    if (c < 0.0) {
+        c = -c;
+        s = -s;
+    }
+    m[0] = -s / (1.0 + c);
+    m[1] = g.kind == LLSV_PG_RX ? -s : s;
+}
+
 // table[b * tstride + 8 * j] = forward 2x2 of one-qubit gate coef_gate[j] for batch element b,
-// table[b * tstride + 8 * (ncoef + j)] = its inverse.
+// table[b * tstride + 8 * (ncoef + j)] = its inverse (RX / RY: with their shears).
 __global__ __launch_bounds__(256) void k_pc_table(const PG* __restrict__ gates, const int* __restrict__ coef_gate,
                                                   int ncoef, const double* __restrict__ params, int P, int B,
                                                   double* __restrict__ table, uint64_t tstride) {
@@ -451,6 +467,10 @@ __global__ __launch_bounds__(256) void k_pc_table(const PG* __restrict__ gates, 
     double* out = table + uint64_t(b) * tstride;
     gate_matrix(g, params, P, b, false, out + 8 * j);
     gate_matrix(g, params, P, b, true, out + 8 * (ncoef + j));
+    if (g.kind == LLSV_PG_RX || g.kind == LLSV_PG_RY) {
+        rotation_shears(g, out + 8 * j);
+        rotation_shears(g, out + 8 * (ncoef + j));
+    }
 }
 
 struct TiledPass {

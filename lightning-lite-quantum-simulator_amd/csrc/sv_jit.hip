@@ -38,7 +38,28 @@ const char kPrelude[] =
     "__device__ __forceinline__ unsigned int r_swzm(unsigned int l, unsigned int r0, unsigned int r1,"
     " unsigned int r2, unsigned int r3) { return l ^ ((__builtin_popcount(l & r0) & 1u) |"
     " ((__builtin_popcount(l & r1) & 1u) << 1) | ((__builtin_popcount(l & r2) & 1u) << 2) |"
-    " ((__builtin_popcount(l & r3) & 1u) << 3)); }\n" LLSV_STRINGIZE(LLSV_REGOPS_SOURCE) "\n";
+    " ((__builtin_popcount(l & r3) & 1u) << 3)); }\n"
+    // r_wsum: the sum over the wavefront's 64 lanes in a fixed order, by DPP moves (no LDS round
+    // trips): pairs, quads, eights, rows of 16, then rows 0+1 into row 1 and rows 0..3 into lane 63
+    // (r_dpp: every lane reads a valid source lane; r_dppz: rows outside RM read 0)
+    "template <int C> __device__ __forceinline__ double r_dpp(double v) {"
+    " const unsigned long long u = __builtin_bit_cast(unsigned long long, v);"
+    " const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)u, C, 0xf, 0xf, false);"
+    " const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), C, 0xf, 0xf, false);"
+    " return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo); }\n"
+    "template <int C, int RM> __device__ __forceinline__ double r_dppz(double v) {"
+    " const unsigned long long u = __builtin_bit_cast(unsigned long long, v);"
+    " const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, C, RM, 0xf, false);"
+    " const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), C, RM, 0xf, false);"
+    " return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo); }\n"
+    "__device__ __forceinline__ double r_wsum(double v) {"
+    " v += r_dpp<0xB1>(v); v += r_dpp<0x4E>(v); v += r_dpp<0x141>(v); v += r_dpp<0x140>(v);"
+    " v += r_dppz<0x142, 0xA>(v); v += r_dppz<0x143, 0xC>(v);"
+    " const unsigned long long u = __builtin_bit_cast(unsigned long long, v);"
+    " const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, 63);"
+    " const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), 63);"
+    " return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo); }\n"
+    LLSV_STRINGIZE(LLSV_REGOPS_SOURCE) "\n";
 
 // ---- LDS slot swizzle of a generated pass ----------------------------------------------------
 // A layout change writes every amplitude of the tile into LDS and reads it back in the new
@@ -308,6 +329,15 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
                                 o.line("r_sc(%s[%d], m[%d], m[%d]);", arrs[k], j, (op.rmask >> j) & 1u ? 6 : 0,
                                        (op.rmask >> j) & 1u ? 7 : 1);
                     }
+                } else if ((form == 1 || form == 2) && op.kind == RO_1QT) {
+                    // table coefficients of RX / RY are the shears of the angle folded into
+                    // [-pi/2, pi/2] (the sign this drops is the same on psi and lambda)
+                    o.line("const double sa = m[0], sb = m[1];");
+                    for (int k = 0; k < na; ++k)
+                        for (int j = 0; j < 16; ++j)
+                            if (!(j & K))
+                                o.line("%s(%s[%d], %s[%d], sa, sb);", form == 1 ? "r_shx" : "r_shy", arrs[k], j, arrs[k],
+                                       j | K);
                 } else if (form == 1 || form == 2) {
                     o.line("const double p = m[0], q = m[%d];", form == 1 ? 3 : 4);
                     for (int k = 0; k < na; ++k)
@@ -374,8 +404,7 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
                     }
                     o.line("double g = g0 + g1;");
                 }
-                o.line("#pragma unroll");
-                o.line("for (int o = 32; o > 0; o >>= 1) g += __shfl_xor(g, o, 64);");
+                o.line("g = r_wsum(g);");
                 o.line("if ((tid & 63u) == 0u) gs[%d][tid >> 6] = g; }", gi);
                 ++gi;
                 break;

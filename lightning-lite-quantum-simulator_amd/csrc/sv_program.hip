@@ -1000,16 +1000,29 @@ std::vector<RegOp> to_register_program(const std::vector<PassOp>& in, int L, int
                 break;
             }
         if (pick < 0) {
+            // A CNOT whose control is a register bit too is a renaming of registers (no
+            // instructions); with the control on a lane bit it is a select per register word.  So
+            // after the bits ops need, a CNOT's control joins the set where there is room.
+            auto control_of = [&](size_t k, uint32_t set) -> uint32_t {
+                const PassOp& po = in[k];
+                return (po.kind == P_CNOT && ((set >> po.b) & 1u)) ? 1u << po.a : 0u;
+            };
             uint32_t ns = 0;
             for (int i : ready) {
                 const uint32_t nb = need[size_t(i)];
                 if (__builtin_popcount(ns | nb) <= kRegBits) ns |= nb;
                 if (__builtin_popcount(ns) == kRegBits) break;
             }
+            for (int i : ready) {
+                const uint32_t cb = control_of(size_t(i), ns);
+                if (__builtin_popcount(ns | cb) <= kRegBits) ns |= cb;
+            }
             for (size_t k = 0; k < N && __builtin_popcount(ns) < kRegBits; ++k) {
                 if (done[k]) continue;
                 const uint32_t nb = need[k];
                 if (__builtin_popcount(ns | nb) <= kRegBits) ns |= nb;
+                const uint32_t cb = control_of(k, ns);
+                if (__builtin_popcount(ns | cb) <= kRegBits) ns |= cb;
             }
             for (int bit = kRegTileBits - 1; bit >= 0 && __builtin_popcount(ns) < kRegBits; --bit)
                 if ((S >> bit) & 1u) ns |= 1u << bit;

@@ -19,6 +19,10 @@
 //   the components go through scalar locals (a bit cast of a vector element expression such
 //   as v.y reads element 0 under this compiler).
 // r_phase: (sr + i si) <- (sr + i si)(dr + i di).
+// r_shy / r_shx: the rotation [[cos f, -sin f], [sin f, cos f]] (RY form) or
+//   [[cos f, -i sin f], [-i sin f, cos f]] (RX form) of an angle |f| <= pi/2 as three shears,
+//   x += a y, y += b x, x += a y (the RX form with i a, i b), with a = -tan(f/2) and b = sin f
+//   (RY) or -sin f (RX): six FMAs per pair, in place.
 // r_nu0 / r_nu1: a one-qubit matrix with its row scales taken out (normalised form), one FMA
 //   per product: r_nu0 (x, y) <- (x + b y, c x + y), r_nu1 (x, y) <- (b x + y, x + c y), with
 //   b = m[0] + i m[1], c = m[2] + i m[3].
@@ -68,6 +72,22 @@
         const double nr = __builtin_fma(sr, dr, -si * di);                                              \
         si = __builtin_fma(sr, di, si * dr);                                                            \
         sr = nr;                                                                                        \
+    }                                                                                                   \
+    __device__ __forceinline__ void r_shy(v2d& x, v2d& y, double a, double b) {                         \
+        x.x = __builtin_fma(a, y.x, x.x);                                                               \
+        x.y = __builtin_fma(a, y.y, x.y);                                                               \
+        y.x = __builtin_fma(b, x.x, y.x);                                                               \
+        y.y = __builtin_fma(b, x.y, y.y);                                                               \
+        x.x = __builtin_fma(a, y.x, x.x);                                                               \
+        x.y = __builtin_fma(a, y.y, x.y);                                                               \
+    }                                                                                                   \
+    __device__ __forceinline__ void r_shx(v2d& x, v2d& y, double a, double b) {                         \
+        x.x = __builtin_fma(-a, y.y, x.x);                                                              \
+        x.y = __builtin_fma(a, y.x, x.y);                                                               \
+        y.x = __builtin_fma(-b, x.y, y.x);                                                              \
+        y.y = __builtin_fma(b, x.x, y.y);                                                               \
+        x.x = __builtin_fma(-a, y.y, x.x);                                                              \
+        x.y = __builtin_fma(a, y.x, x.y);                                                               \
     }                                                                                                   \
     __device__ __forceinline__ void r_nu0(v2d& x, v2d& y, double br, double bi, double cr, double ci) { \
         const double xr = x.x, xi = x.y, yr = y.x, yi = y.y;                                            \
