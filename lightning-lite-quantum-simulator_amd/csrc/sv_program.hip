@@ -989,6 +989,7 @@ std::vector<RegOp> to_register_program(const std::vector<PassOp>& in, int L, int
     for (size_t i = 0; i < N; ++i)
         if (!npred[i]) ready.push_back(int(i));
     std::vector<char> done(N, 0);
+    std::vector<int> work, undo;  // scratch of the lookahead
     std::vector<RegOp> out;
     out.reserve(N + 8);
     uint32_t S = kStart;
@@ -1027,6 +1028,36 @@ std::vector<RegOp> to_register_program(const std::vector<PassOp>& in, int L, int
             for (int bit = kRegTileBits - 1; bit >= 0 && __builtin_popcount(ns) < kRegBits; --bit)
                 if ((S >> bit) & 1u) ns |= 1u << bit;
             for (int bit = kRegTileBits - 1; bit >= 0 && __builtin_popcount(ns) < kRegBits; --bit) ns |= 1u << bit;
+            // Lookahead: every 4-bit set is scored by the ops it lets run before the next change
+            // (a CNOT whose control stays on a lane bit counts half, ops that need no register bit
+            // a quarter); the set built above is the default, another must score higher.
+            auto score = [&](uint32_t cand) {
+                double sc = 0.0;
+                work.assign(ready.begin(), ready.end());
+                undo.clear();
+                for (size_t w = 0; w < work.size(); ++w) {
+                    const size_t i = size_t(work[w]);
+                    if (need[i] & ~cand) continue;
+                    const PassOp& po = in[i];
+                    sc += !need[i] ? 0.25 : (po.kind == P_CNOT && !((cand >> po.a) & 1u)) ? 0.5 : 1.0;
+                    for (int sx : succ[i]) {
+                        undo.push_back(sx);
+                        if (--npred[size_t(sx)] == 0) work.push_back(sx);
+                    }
+                }
+                for (int sx : undo) ++npred[size_t(sx)];
+                return sc;
+            };
+            double best = score(ns);
+            for (uint32_t c = 0xFu; c < (1u << kRegTileBits);) {
+                const double sc = score(c);
+                if (sc > best) {
+                    best = sc;
+                    ns = c;
+                }
+                const uint32_t lo = c & (~c + 1u), r = c + lo;  // next mask with 4 bits set
+                c = (((r ^ c) >> 2) / lo) | r;
+            }
             out.push_back(layout(S, ns));
             S = ns;
             continue;
