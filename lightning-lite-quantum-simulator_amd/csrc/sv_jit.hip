@@ -637,8 +637,8 @@ std::unordered_map<std::string, std::unique_ptr<JitEntry>>& cache() {
 // The process keeps its own cache of compiled passes; keep the code-object manager from writing
 // one to disk.  Set once when the library is loaded, before it starts any thread (setenv must not
 // race with getenv on another thread, and the compiler worker reads the environment).
-// LLSV_JIT_DUMP=<directory>: every compile batch's source is also written there, as
-// <first kernel name>.hip, for reading the generated passes (and compiling them by hand).
+// LLSV_JIT_DUMP=<directory>: every compile batch's source and code object are also written
+// there, as <first kernel name>.hip and .co, for reading the generated passes.
 std::string* g_dump_dir = nullptr;
 __attribute__((constructor)) void jit_env_init() {
     setenv("AMD_COMGR_CACHE", "0", 0);
@@ -646,11 +646,11 @@ __attribute__((constructor)) void jit_env_init() {
     if (d && *d) g_dump_dir = new std::string(d);
 }
 
-void dump_source(const std::string& name, const std::string& src) {
+void dump_file(const std::string& name, const char* data, size_t size) {
     if (!g_dump_dir) return;
-    const std::string path = *g_dump_dir + "/" + name + ".hip";
-    if (FILE* f = std::fopen(path.c_str(), "w")) {
-        std::fwrite(src.data(), 1, src.size(), f);
+    const std::string path = *g_dump_dir + "/" + name;
+    if (FILE* f = std::fopen(path.c_str(), "wb")) {
+        std::fwrite(data, 1, size, f);
         std::fclose(f);
     }
 }
@@ -668,7 +668,7 @@ void compile_batch(const std::vector<JitEntry*>& batch) {
         body.replace(body.find("LLSV_KERNEL"), 11, names[i]);
         src += body;
     }
-    dump_source(names.front(), src);
+    dump_file(names.front() + ".hip", src.data(), src.size());
     auto obj = std::make_shared<CodeObject>();
     std::string error;
     hiprtcProgram prog = nullptr;
@@ -696,6 +696,7 @@ void compile_batch(const std::vector<JitEntry*>& batch) {
         (void)hiprtcDestroyProgram(&prog);
     }
     if (r != HIPRTC_SUCCESS) error = std::string("hipRTC: ") + hiprtcGetErrorString(r) + (error.empty() ? "" : ": " + error);
+    else dump_file(names.front() + ".co", obj->code.data(), obj->code.size());
     std::lock_guard<std::mutex> lock(g_mu);
     for (size_t i = 0; i < batch.size(); ++i) {
         JitEntry* e = batch[i];
