@@ -328,6 +328,18 @@ int llsv_traj_plan_destroy(llsv_traj_plan* p);
  * fused): threads in {256, 512, 1024}, per_cu in {1, 2, 4}, threads * per_cu <= 2048. */
 int llsv_traj_default_config(int threads, int per_cu, int fused);
 
+/* The tuner's choices (batch shape -> threads, workgroups per CU, fused) as a JSON file, so a
+ * later process skips the tuning runs: load adds a file's entries to the process cache (entries
+ * for the same shape are replaced; a malformed file adds nothing and is LLSV_ERR_ARG); save writes
+ * the cache; entries counts it.  With LLSV_TUNE_CACHE=<file> in the environment the library loads
+ * that file on first use and rewrites it after every new choice; LLSV_TRAJ_CONFIG=t,p,f sets the
+ * default configuration and makes plans use it instead of tuning.  No counterpart in the
+ * reference: its CPU loops have no launch configuration, only compile-time constants and
+ * OMP_NUM_THREADS (src/StateVectorBackend.h). */
+int llsv_traj_tune_load(const char* path);
+int llsv_traj_tune_save(const char* path);
+int llsv_traj_tune_entries(void);
+
 /* ---- zero-noise extrapolation: folded circuits under sampled depolarizing noise ---- */
 /* The noise model of ll::NoisyBackend (src/Mitigation.h:196-303): every logical gate U runs as
  * U (U^+ U)^k for scale 2k + 1, and after each application the touched qubits get depolarizing

@@ -20,8 +20,11 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
+#include <string>
 #include <memory>
 #include <mutex>
 #include <numeric>
@@ -493,11 +496,137 @@ struct TrajShape {
     }
 };
 
+bool valid_config(const TrajConfig& c) {
+    return (c.wg == 256 || c.wg == 512 || c.wg == 1024) && (c.per_cu == 1 || c.per_cu == 2 || c.per_cu == 4) &&
+           (c.fuse == 0 || c.fuse == 1) && c.wg * c.per_cu <= 2048;
+}
+
+// ---- tuning cache --------------------------------------------------------------------------
+// Choices of the tuner, per batch shape, for the process; with LLSV_TUNE_CACHE=<file> also kept
+// on disk as JSON (read on first use, rewritten after every new choice), so later processes skip
+// the tuning runs.  The file is a list of flat objects:
+//   {"format": "llsv-traj-tune-1", "entries": [
+//     {"maxn": 14, "minn": 14, "lg_items": 12, "lg_gates": 8, "threads": 512, "per_cu": 2, "fused": 1}, ...]}
+// LLSV_TRAJ_CONFIG=<threads>,<per_cu>,<fused> (e.g. 512,2,1) sets the default configuration and
+// takes the tuner's place: plans use it instead of tuning.
 std::mutex g_tune_mu;
 TrajConfig g_default;  // used when no tuned choice is cached (guarded by g_tune_mu)
-std::map<TrajShape, TrajConfig>& tune_cache() {
+bool g_forced = false;  // LLSV_TRAJ_CONFIG given: no tuning
+std::map<TrajShape, TrajConfig>& tune_cache_raw() {
     static auto* m = new std::map<TrajShape, TrajConfig>();
     return *m;
+}
+
+// The integer after "key": in [b, e), or fails.
+bool json_int(const char* b, const char* e, const char* key, int* out) {
+    const std::string k = std::string("\"") + key + "\"";
+    const char* p = std::search(b, e, k.begin(), k.end());
+    if (p == e) return false;
+    p += k.size();
+    while (p < e && (*p == ' ' || *p == ':' || *p == '\t' || *p == '\n' || *p == '\r')) ++p;
+    char* end = nullptr;
+    const long v = std::strtol(p, &end, 10);
+    if (end == p || end > e) return false;
+    *out = int(v);
+    return true;
+}
+
+// Parses a cache file's text; false (and nothing added) on any malformed entry.
+bool parse_tune_json(const std::string& text, std::map<TrajShape, TrajConfig>* into) {
+    if (text.find("\"llsv-traj-tune-1\"") == std::string::npos) return false;
+    const size_t ent = text.find("\"entries\"");
+    if (ent == std::string::npos) return false;
+    std::map<TrajShape, TrajConfig> got;
+    size_t pos = text.find('[', ent);
+    if (pos == std::string::npos) return false;
+    for (;;) {
+        const size_t ob = text.find('{', pos), close = text.find(']', pos);
+        if (ob == std::string::npos || (close != std::string::npos && close < ob)) break;
+        const size_t cb = text.find('}', ob);
+        if (cb == std::string::npos) return false;
+        const char *b = text.data() + ob, *e = text.data() + cb;
+        TrajShape sh{};
+        TrajConfig c{};
+        if (!json_int(b, e, "maxn", &sh.maxn) || !json_int(b, e, "minn", &sh.minn) ||
+            !json_int(b, e, "lg_items", &sh.lg_items) || !json_int(b, e, "lg_gates", &sh.lg_gates) ||
+            !json_int(b, e, "threads", &c.wg) || !json_int(b, e, "per_cu", &c.per_cu) || !json_int(b, e, "fused", &c.fuse))
+            return false;
+        if (!valid_config(c) || sh.minn < 1 || sh.maxn > 20 || sh.minn > sh.maxn) return false;
+        got[sh] = c;
+        pos = cb + 1;
+    }
+    for (const auto& kv : got) (*into)[kv.first] = kv.second;
+    return true;
+}
+
+std::string tune_json(const std::map<TrajShape, TrajConfig>& m) {
+    std::string s = "{\"format\": \"llsv-traj-tune-1\", \"entries\": [";
+    bool first = true;
+    for (const auto& kv : m) {
+        char buf[256];
+        snprintf(buf, sizeof buf,
+                 "%s\n  {\"maxn\": %d, \"minn\": %d, \"lg_items\": %d, \"lg_gates\": %d, \"threads\": %d, "
+                 "\"per_cu\": %d, \"fused\": %d}",
+                 first ? "" : ",", kv.first.maxn, kv.first.minn, kv.first.lg_items, kv.first.lg_gates, kv.second.wg,
+                 kv.second.per_cu, kv.second.fuse);
+        s += buf;
+        first = false;
+    }
+    s += "\n]}\n";
+    return s;
+}
+
+bool read_file(const char* path, std::string* out) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return false;
+    char buf[4096];
+    size_t k;
+    out->clear();
+    while ((k = std::fread(buf, 1, sizeof buf, f)) > 0) out->append(buf, k);
+    std::fclose(f);
+    return true;
+}
+
+// Writes through a temporary file and a rename, so a reader never sees half a file.
+bool write_file(const char* path, const std::string& text) {
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return false;
+    const bool ok = std::fwrite(text.data(), 1, text.size(), f) == text.size();
+    if (std::fclose(f) != 0 || !ok) return false;
+    return std::rename(tmp.c_str(), path) == 0;
+}
+
+const char* cache_path() {
+    static const char* p = [] {
+        const char* e = std::getenv("LLSV_TUNE_CACHE");
+        return (e && *e) ? e : nullptr;
+    }();
+    return p;
+}
+
+// The cache (caller holds g_tune_mu); the environment is read on first use.
+std::map<TrajShape, TrajConfig>& tune_cache() {
+    static bool init = false;
+    if (!init) {
+        init = true;
+        std::string text;
+        if (cache_path() && read_file(cache_path(), &text)) (void)parse_tune_json(text, &tune_cache_raw());
+        if (const char* e = std::getenv("LLSV_TRAJ_CONFIG")) {
+            TrajConfig c{};
+            if (std::sscanf(e, "%d,%d,%d", &c.wg, &c.per_cu, &c.fuse) == 3 && valid_config(c)) {
+                g_default = c;
+                g_forced = true;
+            }
+        }
+    }
+    return tune_cache_raw();
+}
+
+// Records a new choice; rewrites the cache file (caller holds g_tune_mu).
+void tune_store(const TrajShape& sh, const TrajConfig& c) {
+    tune_cache()[sh] = c;
+    if (cache_path()) (void)write_file(cache_path(), tune_json(tune_cache()));
 }
 
 int ilog2(uint64_t x) { return x ? 63 - __builtin_clzll(x) : 0; }
@@ -684,11 +813,11 @@ extern "C" int llsv_traj_run(const llsv_traj_item* items, int n_items, const lls
     TrajShape shape{};
     if (int rc = check_batch(items, n_items, gates, n_gates, gamma, lambda, n_rates, "llsv_traj_run", &order, &shape))
         return rc;
-    TrajConfig cfg;  // a configuration the tuner chose for this shape earlier in the process, else the default
+    TrajConfig cfg;  // a configuration the tuner chose for this shape (this process or the cache file), else the default
     {
         std::lock_guard<std::mutex> lock(g_tune_mu);
         auto it = tune_cache().find(shape);
-        cfg = it != tune_cache().end() ? it->second : g_default;
+        cfg = (it != tune_cache().end() && !g_forced) ? it->second : g_default;
     }
     const int cus = device_cus();
     const int blocks = traj_blocks(cfg, n_items, shape.maxn, cus);
@@ -758,7 +887,9 @@ extern "C" int llsv_traj_plan_create(const llsv_traj_item* items, int n_items, c
         std::lock_guard<std::mutex> lock(g_tune_mu);
         auto it = tune_cache().find(shape);
         p->cfg = g_default;
-        if (tune >= 0 && it != tune_cache().end()) {
+        if (g_forced) {
+            cached = true;  // LLSV_TRAJ_CONFIG: the given configuration, no tuning
+        } else if (tune >= 0 && it != tune_cache().end()) {
             p->cfg = it->second;
             cached = true;
         }
@@ -766,7 +897,7 @@ extern "C" int llsv_traj_plan_create(const llsv_traj_item* items, int n_items, c
     if (tune == 1 && !cached) {
         if (int rc = traj_tune(p->dev, items, n_items, order, shape.maxn, p->stream, &p->cfg, &p->tune_ms)) return rc;
         std::lock_guard<std::mutex> lock(g_tune_mu);
-        tune_cache()[shape] = p->cfg;
+        tune_store(shape, p->cfg);
     }
     p->blocks = std::min(traj_blocks(p->cfg, n_items, shape.maxn, cus), max_blocks);
     if (dev_host_alloc(reinterpret_cast<void**>(&p->h_seed), 256) != hipSuccess ||
@@ -803,13 +934,37 @@ extern "C" int llsv_traj_plan_create(const llsv_traj_item* items, int n_items, c
 
 extern "C" int llsv_traj_default_config(int threads, int per_cu, int fused) {
     using namespace llsv;
-    if ((threads != 256 && threads != 512 && threads != 1024) || (per_cu != 1 && per_cu != 2 && per_cu != 4) ||
-        (fused != 0 && fused != 1) || threads * per_cu > 2048)
+    if (!valid_config(TrajConfig{threads, per_cu, fused}))
         return fail(LLSV_ERR_ARG, "llsv_traj_default_config: threads in {256, 512, 1024}, per_cu in {1, 2, 4}, "
                                   "threads * per_cu <= 2048, fused in {0, 1}");
     std::lock_guard<std::mutex> lock(g_tune_mu);
+    (void)tune_cache();  // the environment's settings first, so this call overrides them
     g_default = TrajConfig{threads, per_cu, fused};
     return LLSV_OK;
+}
+
+extern "C" int llsv_traj_tune_load(const char* path) {
+    using namespace llsv;
+    if (!path) return fail(LLSV_ERR_ARG, "llsv_traj_tune_load: path is null");
+    std::string text;
+    if (!read_file(path, &text)) return fail(LLSV_ERR_ARG, "llsv_traj_tune_load: cannot read %s", path);
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    if (!parse_tune_json(text, &tune_cache()))
+        return fail(LLSV_ERR_ARG, "llsv_traj_tune_load: %s is not a tuning cache (llsv-traj-tune-1)", path);
+    return LLSV_OK;
+}
+
+extern "C" int llsv_traj_tune_save(const char* path) {
+    using namespace llsv;
+    if (!path) return fail(LLSV_ERR_ARG, "llsv_traj_tune_save: path is null");
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    if (!write_file(path, tune_json(tune_cache()))) return fail(LLSV_ERR_ARG, "llsv_traj_tune_save: cannot write %s", path);
+    return LLSV_OK;
+}
+
+extern "C" int llsv_traj_tune_entries(void) {
+    std::lock_guard<std::mutex> lock(llsv::g_tune_mu);
+    return int(llsv::tune_cache().size());
 }
 
 extern "C" int llsv_traj_plan_run(llsv_traj_plan* p, uint64_t seed, double* z, uint32_t* jumps, uint64_t* hashes) {

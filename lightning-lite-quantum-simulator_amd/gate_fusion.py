@@ -18,6 +18,8 @@ an op joins the open block while the union of qubits stays within ``max_k``; oth
 the block is emitted and a new one opens.  The emitted circuit equals the input one up to
 rounding.
 
+``plan_fusion(n, ops, max_k=None)`` also chooses the fusion width (2..5) by the same prediction.
+
 ``plan_fusion(n, ops)`` chooses, region by region, between the two ways the engine runs a
 stretch of circuit: one register-tile pass of the program path (``apply_program``: one read
 and write of the state, every gate in FP64 vector arithmetic) or dense blocks of at most five
@@ -197,15 +199,17 @@ def fuse_dense(ops, max_k: int = 5):
 # of a register-tile pass at 3.93e13 per second (78.6 TFLOP/s of FMAs); a dense block of k qubits
 # at the bandwidth its kernel reaches (k = 4, 5: FP64 matrix cores, 5.1 / 4.8 TB/s; k <= 3:
 # vector arithmetic).  A register-tile pass costs its memory time plus its FP64 issue time (the
-# two overlap little at two workgroups per CU): this reproduces the 30-qubit sweep's passes (7.7 ms
-# predicted, 7.5 measured) and the 28-qubit layered circuit's (3.2 vs 3.8 ms).
+# two overlap little at two workgroups per CU): this reproduces the 28-qubit layered circuit's 11
+# passes (31.0 ms predicted, 30.2 measured, each pass within 20%) and the 30-qubit sweep's 4
+# (25.8 vs 24.9 ms).
 PASS_BPS = 6.2e12
 FP64_INSTR_PER_S = 3.93e13
 BLOCK_BPS = {1: 6.2e12, 2: 5.5e12, 3: 5.0e12, 4: 5.1e12, 5: 4.8e12}
-# FP64 vector instructions per amplitude for the ops of a register-tile pass: a general 2x2
-# (RX RY RZ merged, Haar, H) 8; an RX or RY alone 4; a diagonal one-qubit gate 2; CNOT, CZ and
-# MCZ move or negate amplitudes (about 0.5).
-_OPS_GENERAL, _OPS_ROT, _OPS_DIAG, _OPS_PERM = 8.0, 4.0, 2.0, 0.5
+# FP64 vector instructions per amplitude for the ops of a register-tile pass: a one-qubit matrix
+# (RX RY RZ merged, Haar, H, an RX or RY alone) 4, one FMA per product in its normalised form;
+# a diagonal one-qubit gate is carried into the next matrix on its qubit and costs about 1 (where a
+# CNOT target or a pass end applies it); CNOT, CZ and MCZ move or negate amplitudes (about 0.5).
+_OPS_GENERAL, _OPS_ROT, _OPS_DIAG, _OPS_PERM = 4.0, 4.0, 1.0, 0.5
 
 
 def _fp64_per_amp(ops) -> float:
@@ -291,13 +295,22 @@ def _groups(ops, max_k):
     return out
 
 
-def plan_fusion(n: int, ops, max_k: int = 5, tile_bits: int = 12):
+def plan_fusion(n: int, ops, max_k: int | None = 5, tile_bits: int = 12):
     """Segments [("program", ops) | ("dense", blocks)] that apply the circuit, and their predicted
     seconds.  The circuit is cut into fuse_dense's groups (at most max_k qubits each); a group of two
     or more qubits becomes one dense block when its FP64 work inside a register-tile pass would take
     longer than a dense pass over the state, and the other groups stay gates for the program path
     (consecutive ones in one segment, which the planner packs into passes).  If the program path
-    alone is predicted faster than that plan, the plan is the whole circuit as one program."""
+    alone is predicted faster than that plan, the plan is the whole circuit as one program.
+    max_k=None chooses the fusion width as well: the plan of lowest predicted time over widths
+    2..5 (the widest on a tie)."""
+    if max_k is None:
+        best = None
+        for k in (5, 4, 3, 2):
+            cand = plan_fusion(n, ops, k, tile_bits)
+            if best is None or cand[1] < best[1]:
+                best = cand
+        return best
     segs = []
     for targets, g in _groups(ops, max_k):
         k = len(targets)

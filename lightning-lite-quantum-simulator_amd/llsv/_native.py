@@ -76,6 +76,9 @@ SIGNATURES = {
     "llsv_traj_plan_config": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_D)]),
     "llsv_traj_plan_destroy": (_I, [_P]),
     "llsv_traj_default_config": (_I, [_I, _I, _I]),
+    "llsv_traj_tune_load": (_I, [ctypes.c_char_p]),
+    "llsv_traj_tune_save": (_I, [ctypes.c_char_p]),
+    "llsv_traj_tune_entries": (_I, []),
     "llsv_mark": (_I, [_P, _I]),
     "llsv_mark_elapsed": (_I, [_P, _I, _I, ctypes.POINTER(ctypes.c_float)]),
     "llsv_device_ptr": (_P, [_P]),

@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import os
 
 import numpy as np
 
@@ -69,6 +70,21 @@ def _pack_batch(circuits):
 def set_default_config(threads: int, per_cu: int, fused: bool):
     """Process-wide default launch configuration (llsv_traj_default_config)."""
     _native.check(_native.lib().llsv_traj_default_config(threads, per_cu, int(bool(fused))))
+
+
+def load_tuning(path: str):
+    """Adds the tuner choices stored in a JSON file to the process cache (llsv_traj_tune_load)."""
+    _native.check(_native.lib().llsv_traj_tune_load(os.fsencode(path)))
+
+
+def save_tuning(path: str):
+    """Writes the process's tuner choices to a JSON file (llsv_traj_tune_save)."""
+    _native.check(_native.lib().llsv_traj_tune_save(os.fsencode(path)))
+
+
+def tuning_entries() -> int:
+    """Number of batch shapes with a tuned configuration in the process cache."""
+    return int(_native.lib().llsv_traj_tune_entries())
 
 
 class TrajectoryPlan:

@@ -100,11 +100,32 @@ def test_plan_fusion_choices_and_equivalence():
     assert t < GF.predict_blocks(28, GF.fuse_dense(lay, 5))
     # mixed, on 8 qubits where the oracle runs it: the segments apply the same circuit
     n = 8
-    mixed = deep_few_qubits(6, [0, 2, 5], 120) + C.layered_rotations(2, n, 2) + deep_few_qubits(7, [1, 4, 6, 7], 90)
+    mixed = deep_few_qubits(6, [0, 2, 5], 400) + C.layered_rotations(2, n, 2) + deep_few_qubits(7, [1, 4, 6, 7], 300)
     segs, t = GF.plan_fusion(n, mixed)
     assert {k for k, _ in segs} == {"dense", "program"} and t <= GF.predict_program(n, mixed)
     got = O.run(n, _segments_as_ops(segs))
     assert np.max(np.abs(got - O.run(n, mixed))) < 1e-12
     # the predicted pass time: memory plus FP64 issue, same-qubit one-qubit runs merged
     assert GF.predict_pass(20, [("rx", 0, 0.1), ("ry", 0, 0.2), ("cz", 0, 1), ("rz", 3, 0.4)]) == \
-        32.0 * 2**20 / GF.PASS_BPS + (8 + 0.5 + 2) * 2**20 / GF.FP64_INSTR_PER_S
+        32.0 * 2**20 / GF.PASS_BPS + (GF._OPS_GENERAL + GF._OPS_PERM + GF._OPS_DIAG) * 2**20 / GF.FP64_INSTR_PER_S
+
+
+def test_plan_fusion_chooses_width():
+    import gate_fusion as GF
+    from llsv import circuits as C
+    from oracle import oracle as O
+
+    # max_k=None: the plan of lowest predicted time over widths 2..5
+    for n, ops in [(28, deep_few_qubits(5, [3, 7, 11, 19, 25], 300)), (28, C.layered_rotations(1, 28, 4)),
+                   (9, deep_few_qubits(8, [0, 1, 2, 3], 80) + deep_few_qubits(9, [4, 5, 6, 7, 8], 80))]:
+        segs, t = GF.plan_fusion(n, ops, None)
+        widths = {k: GF.plan_fusion(n, ops, k)[1] for k in (2, 3, 4, 5)}
+        assert t == min(widths.values())
+    # a five-qubit register: only width 5 makes it one block, which the choice takes
+    deep = deep_few_qubits(5, [3, 7, 11, 19, 25], 300)
+    segs, t = GF.plan_fusion(28, deep, None)
+    assert [k for k, _ in segs] == ["dense"] and len(segs[0][1][0][1]) == 5
+    # the chosen plan applies the same circuit (oracle, 9 qubits)
+    ops = deep_few_qubits(8, [0, 1, 2, 3], 80) + deep_few_qubits(9, [4, 5, 6, 7, 8], 80)
+    segs, _ = GF.plan_fusion(9, ops, None)
+    assert np.max(np.abs(O.run(9, _segments_as_ops(segs)) - O.run(9, ops))) < 1e-12

@@ -179,3 +179,49 @@ def test_tuned_plan_mixed_batch():
     assert np.array_equal(j, jr) and np.array_equal(h, hr)
     assert np.max(np.abs(z - zr)) < 1e-12
     plan.close()
+
+
+_PLAN_CHILD = """
+import sys
+sys.path[:0] = [{root!r}, {pkg!r}]
+import numpy as np
+from llsv import circuits as C, trajectory as LT
+n, count = 10, 384
+gates, _, _ = C.noisy_brick(9, n, 5)
+gamma = np.random.default_rng(9).uniform(1e-3, 2e-2, n)
+plan = LT.TrajectoryPlan.from_circuit(n, gates, gamma, gamma / 2, count, tune=True)
+t, w, f, ms = plan.config()
+z, j, h = plan.run(5)
+zr, jr, hr = LT.run_trajectories(n, gates, gamma, gamma / 2, seed=5, n_traj=count)
+assert np.array_equal(j, jr) and np.array_equal(h, hr) and np.max(np.abs(z - zr)) < 1e-12
+print(t, w, int(f), ms)
+"""
+
+
+def _child(env_extra):
+    import subprocess
+
+    from conftest import PKG, ROOT
+
+    env = dict(os.environ, **env_extra)
+    r = subprocess.run([sys.executable, "-c", _PLAN_CHILD.format(root=ROOT, pkg=PKG)], env=env, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    t, w, f, ms = r.stdout.split()[-4:]
+    return int(t), int(w), int(f), float(ms)
+
+
+def test_tuning_cache_file_and_forced_config(tmp_path):
+    # a process that tunes writes LLSV_TUNE_CACHE; the next process reads it and does not time
+    # again; LLSV_TRAJ_CONFIG forces a configuration instead of tuning (results stay the same)
+    cache = tmp_path / "tune.json"
+    say("tuning cache: first process tunes")
+    t1, w1, f1, ms1 = _child({"LLSV_TUNE_CACHE": str(cache)})
+    assert ms1 > 0 and cache.exists()
+    assert json.load(open(cache))["format"] == "llsv-traj-tune-1"
+    say("tuning cache: second process reads the file")
+    t2, w2, f2, ms2 = _child({"LLSV_TUNE_CACHE": str(cache)})
+    assert ms2 == 0.0 and (t2, w2, f2) == (t1, w1, f1)
+    say("forced configuration")
+    t3, _, f3, ms3 = _child({"LLSV_TRAJ_CONFIG": "1024,1,0"})
+    assert ms3 == 0.0 and (t3, f3) == (1024, 0)
