@@ -219,6 +219,62 @@ __device__ inline void pass_two(v2d* s, uint64_t N, int a, const double* ma, int
     }
 }
 
+// Two CNOTs on disjoint qubit pairs, (c0, t0) then (c1, t1), in one pass over groups of 16
+// amplitudes: for each pair in turn, the pending operators of its control and target, then the
+// CNOT -- the arithmetic of two pass_two calls, with one read and write of the state.
+template <int WG>
+__device__ inline void pass_four(v2d* s, uint64_t N, int c0, const double* mc0, int t0, const double* mt0, int c1,
+                                 const double* mc1, int t1, const double* mt1) {
+    int pos[4] = {c0, t0, c1, t1};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3 - i; ++j)
+            if (pos[j] > pos[j + 1]) {
+                const int x = pos[j];
+                pos[j] = pos[j + 1];
+                pos[j + 1] = x;
+            }
+    // amplitude j of a group: bits c0, t0, c1, t1 of the index = bits 0, 1, 2, 3 of j
+    const uint64_t bit[4] = {uint64_t(1) << c0, uint64_t(1) << t0, uint64_t(1) << c1, uint64_t(1) << t1};
+    const double* op[4] = {mc0, mt0, mc1, mt1};
+    for (uint64_t p = threadIdx.x; p < N / 16; p += WG) {
+        const uint64_t i = insert0(insert0(insert0(insert0(p, pos[0]), pos[1]), pos[2]), pos[3]);
+        v2d x[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            x[j] = s[i | ((j & 1) ? bit[0] : 0) | ((j & 2) ? bit[1] : 0) | ((j & 4) ? bit[2] : 0) | ((j & 8) ? bit[3] : 0)];
+#pragma unroll
+        for (int pr = 0; pr < 2; ++pr) {
+            const int K0 = 1 << (2 * pr), K1 = 2 << (2 * pr);  // control bit, target bit of j
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (!(j & K0)) {
+                    const v2d y0 = ap(op[2 * pr], x[j], x[j | K0]), y1 = ap(op[2 * pr] + 4, x[j], x[j | K0]);
+                    x[j] = y0;
+                    x[j | K0] = y1;
+                }
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (!(j & K1)) {
+                    const v2d y0 = ap(op[2 * pr + 1], x[j], x[j | K1]), y1 = ap(op[2 * pr + 1] + 4, x[j], x[j | K1]);
+                    x[j] = y0;
+                    x[j | K1] = y1;
+                }
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if ((j & K0) && !(j & K1)) {  // control set: target flipped
+                    const v2d y = x[j];
+                    x[j] = x[j | K1];
+                    x[j | K1] = y;
+                }
+        }
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            s[i | ((j & 1) ? bit[0] : 0) | ((j & 2) ? bit[1] : 0) | ((j & 4) ? bit[2] : 0) | ((j & 8) ? bit[3] : 0)] = x[j];
+    }
+}
+
 // Bits of the possible-jump mask of a gate: site s may jump only if u_s >= 1 - rate (a jump needs
 // u >= 1 - rate * p with p <= 1, both sides rounded the way decide() rounds them).
 __device__ inline unsigned possible_sites(const TG& g, const double* gamma, const double* lam, int first_rate,
@@ -259,6 +315,10 @@ constexpr int TRAJ_CHUNK = 256;  // gates whose possible-jump masks are drawn to
 //     CNOT's pass) or a possible jump needs weights (then every pending operator is applied, two
 //     qubits per pass, the last pass accumulating the weights); !DEFER: each gate's operators are
 //     applied right after it, one pass per gate (plus one for a CNOT's noise).
+//   * DEFER: a CNOT with no possible jump is held, with a copy of the pending operators of its two
+//     qubits; when the next such CNOT is on two other qubits, both run in one pass (pass_four),
+//     else the held one runs alone first.  Operators pushed on a held CNOT's qubits after it stay
+//     pending, so they apply after it; anything that reads or needs the state runs it first.
 // At a possible jump every thread runs decide() on the same block sums, so the outcome needs no
 // broadcast; the chosen operator, scaled to renormalise, becomes the qubit's pending operator.
 // <Z_q> divides by the final norm.
@@ -325,6 +385,17 @@ __global__ __launch_bounds__(WG) void k_traj(const TItem* __restrict__ items, in
             }
         };
         // s_pend[q] <- a * s_pend[q] (or a when q has nothing pending); thread 0 only.
+        // the held CNOT (DEFER), if any: control hc, target ht, their operators before it
+        bool hold = false;
+        int hc = 0, ht = 0;
+        double hmc[8], hmt[8];
+        auto run_held = [&] {
+            if (!hold) return;
+            __syncthreads();
+            pass_two<WG, false, true>(s, N, hc, hmc, ht, hmt, hc, ht, nullptr);
+            __syncthreads();
+            hold = false;
+        };
         auto push = [&](int q, const double* a) {
             if (tid == 0) {
                 if (pmask & (1u << q))
@@ -356,6 +427,7 @@ __global__ __launch_bounds__(WG) void k_traj(const TItem* __restrict__ items, in
                 } else {
                     push(q, g.m);
                     double v[4] = {0.0, 0.0, 0.0, 0.0}, P4[4];
+                    run_held();
                     flush(pmask, true, q, q, v);
                     pmask = 0;
                     block_sums<WG, 4>(v, lds, P4);
@@ -373,6 +445,34 @@ __global__ __launch_bounds__(WG) void k_traj(const TItem* __restrict__ items, in
             } else {
                 const int c = g.q0, t = g.q1;
                 const uint32_t ct = (1u << c) | (1u << t);
+                if (DEFER && !pm) {
+                    // no possible jump: held for one pass with the next CNOT on two other qubits
+                    __syncthreads();
+                    double mc[8] = {1, 0, 0, 0, 0, 0, 1, 0}, mt[8] = {1, 0, 0, 0, 0, 0, 1, 0};
+                    if (pmask & (1u << c)) copy8(s_pend[c], mc);
+                    if (pmask & (1u << t)) copy8(s_pend[t], mt);
+                    if (hold && hc != c && hc != t && ht != c && ht != t) {
+                        pass_four<WG>(s, N, hc, hmc, ht, hmt, c, mc, t, mt);
+                        __syncthreads();
+                        hold = false;
+                    } else {
+                        run_held();
+                        hold = true;
+                        hc = c;
+                        ht = t;
+                        copy8(mc, hmc);
+                        copy8(mt, hmt);
+                    }
+                    pmask &= ~ct;
+                    double kc[8], kt[8];
+                    no_jump_op(gamma[item.first_rate + c], lam[item.first_rate + c], kc);
+                    no_jump_op(gamma[item.first_rate + t], lam[item.first_rate + t], kt);
+                    __syncthreads();  // every thread has read s_pend[c], s_pend[t] before thread 0 rewrites them
+                    push(c, kc);
+                    push(t, kt);
+                    continue;
+                }
+                run_held();
                 if (pm) {  // weights after the CNOT need every other pending operator applied first
                     flush(pmask & ~ct, false, 0, 0, nullptr);
                     pmask &= ct;
@@ -412,6 +512,7 @@ __global__ __launch_bounds__(WG) void k_traj(const TItem* __restrict__ items, in
                 }
             }
         }
+        run_held();
         flush(pmask, false, 0, 0, nullptr);
         pmask = 0;
         // <Z_q> = sum_i p_i (1 - 2 bit_q(i)) / sum_i p_i, seven qubits per read of the state (few

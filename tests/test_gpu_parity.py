@@ -112,6 +112,22 @@ def test_random_circuits_small(n):
     assert maxdiff(gpu_run(n, ops, program=True), ref) <= TOL
 
 
+@pytest.mark.parametrize("n", [4, 13, 17])
+def test_apply_gate_standard(n):
+    # the reference's mask-loop entry (src/StateVectorOptimized.cpp:81-97) runs the same kernels as
+    # apply_gate_cache_optimized: identical bits, and the oracle's state to 1e-12
+    rng = np.random.default_rng(700 + n)
+    ops = [("u", int(q), random_unitary(rng, 2).reshape(-1).tolist()) for q in rng.integers(0, n, 60)]
+    say(f"apply_gate_standard n={n}")
+    a, b = qsv().StateVector(n), qsv().StateVector(n)
+    for op in ops:
+        a.apply_gate_standard(op[1], op[2])
+        b.apply_gate_cache_optimized(op[1], op[2])
+    sa = a.state()
+    assert np.array_equal(sa, b.state())
+    assert maxdiff(sa, O.run(n, ops)) <= TOL
+
+
 @pytest.mark.parametrize("program", [False, True])
 @pytest.mark.parametrize("n", [18, 21])
 def test_every_target_and_pair(n, program):
@@ -314,7 +330,7 @@ def test_fusion_plan_runs_the_circuit():
     from test_fusion import deep_few_qubits
 
     n = 16
-    mixed = deep_few_qubits(6, [0, 2, 5], 120) + C.layered_rotations(2, n, 2) + deep_few_qubits(7, [1, 4, 6, 7], 90)
+    mixed = deep_few_qubits(6, [0, 2, 5], 400) + C.layered_rotations(2, n, 2) + deep_few_qubits(7, [1, 4, 6, 7], 300)
     segs, _ = GF.plan_fusion(n, mixed)
     assert {k for k, _ in segs} == {"dense", "program"}
     say("fusion plan n=16")
