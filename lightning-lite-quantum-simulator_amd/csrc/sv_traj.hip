@@ -315,10 +315,12 @@ constexpr int TRAJ_CHUNK = 256;  // gates whose possible-jump masks are drawn to
 //     CNOT's pass) or a possible jump needs weights (then every pending operator is applied, two
 //     qubits per pass, the last pass accumulating the weights); !DEFER: each gate's operators are
 //     applied right after it, one pass per gate (plus one for a CNOT's noise).
-//   * DEFER: a CNOT with no possible jump is held, with a copy of the pending operators of its two
-//     qubits; when the next such CNOT is on two other qubits, both run in one pass (pass_four),
-//     else the held one runs alone first.  Operators pushed on a held CNOT's qubits after it stay
-//     pending, so they apply after it; anything that reads or needs the state runs it first.
+//   * DEFER, WG <= 512 (1024-thread workgroups have half the registers per thread and would
+//     spill the 16-amplitude groups): a CNOT with no possible jump is held, with a copy of the
+//     pending operators of its two qubits; when the next such CNOT is on two other qubits, both
+//     run in one pass (pass_four), else the held one runs alone first.  Operators pushed on a
+//     held CNOT's qubits after it stay pending, so they apply after it; anything that reads or
+//     needs the state runs it first.
 // At a possible jump every thread runs decide() on the same block sums, so the outcome needs no
 // broadcast; the chosen operator, scaled to renormalise, becomes the qubit's pending operator.
 // <Z_q> divides by the final norm.
@@ -330,6 +332,7 @@ __global__ __launch_bounds__(WG) void k_traj(const TItem* __restrict__ items, in
                                              uint64_t slot_size, unsigned* __restrict__ counter,
                                              double* __restrict__ zout, unsigned* __restrict__ jout,
                                              uint64_t* __restrict__ hout, int norm_state) {
+    constexpr bool kPair = DEFER && WG <= 512;  // held CNOTs run in pairs (pass_four)
     __shared__ int s_item;
     __shared__ double s_pend[20][8];
     __shared__ unsigned char s_pm[TRAJ_CHUNK];
@@ -445,7 +448,7 @@ __global__ __launch_bounds__(WG) void k_traj(const TItem* __restrict__ items, in
             } else {
                 const int c = g.q0, t = g.q1;
                 const uint32_t ct = (1u << c) | (1u << t);
-                if (DEFER && !pm) {
+                if (kPair && !pm) {
                     // no possible jump: held for one pass with the next CNOT on two other qubits
                     __syncthreads();
                     double mc[8] = {1, 0, 0, 0, 0, 0, 1, 0}, mt[8] = {1, 0, 0, 0, 0, 0, 1, 0};
