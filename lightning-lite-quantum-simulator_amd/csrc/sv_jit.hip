@@ -518,19 +518,25 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
     o.line("{ unsigned long long gt = 0;");
     o.line("#pragma unroll");
     o.line("for (int j = 0; j < 12; ++j) if ((lt >> j) & 1u) gt |= lg[j];");
-    if (scaled) {
-        o.line("if (si != 0.0) {");
+    // one state at a time (scale, then store): scaling both before the stores keeps two versions of
+    // all 32 amplitudes of a two-state pass live across the branch, past the 256-register budget
+    const char* dst[2] = {"psi", "lam"};
+    for (int k = 0; k < na; ++k) {
+        if (scaled) {
+            o.line("if (si != 0.0) {");
+            o.line("#pragma unroll");
+            o.line("for (int j = 0; j < 16; ++j) r_sc(%s[j], sr, si);", arrs[k]);
+            o.line("} else if (sr != 1.0) {");
+            o.line("#pragma unroll");
+            o.line("for (int j = 0; j < 16; ++j) %s[j] *= sr; }", arrs[k]);
+        }
         o.line("#pragma unroll");
-        o.line("for (int j = 0; j < 16; ++j) { r_sc(a[j], sr, si);%s }", sh.two ? " r_sc(l[j], sr, si);" : "");
-        o.line("} else if (sr != 1.0) {");
-        o.line("#pragma unroll");
-        o.line("for (int j = 0; j < 16; ++j) { a[j] *= sr;%s } }", sh.two ? " l[j] *= sr;" : "");
+        o.line("for (int j = 0; j < 16; ++j) { const unsigned long long o = base | gt | ((j & 1) ? lg[%d] : 0ull) | "
+               "((j & 2) ? lg[%d] : 0ull) | ((j & 4) ? lg[%d] : 0ull) | ((j & 8) ? lg[%d] : 0ull);",
+               rq[0], rq[1], rq[2], rq[3]);
+        o.line("__builtin_nontemporal_store(%s[j], %s + o); }", arrs[k], dst[k]);
     }
-    o.line("#pragma unroll");
-    o.line("for (int j = 0; j < 16; ++j) { const unsigned long long o = base | gt | ((j & 1) ? lg[%d] : 0ull) | "
-           "((j & 2) ? lg[%d] : 0ull) | ((j & 4) ? lg[%d] : 0ull) | ((j & 8) ? lg[%d] : 0ull);",
-           rq[0], rq[1], rq[2], rq[3]);
-    o.line("__builtin_nontemporal_store(a[j], psi + o);%s } }", sh.two ? " __builtin_nontemporal_store(l[j], lam + o);" : "");
+    o.line("}");
     if (!gslot.empty()) {
         bool contiguous = true;
         for (size_t k = 0; k < gslot.size(); ++k) contiguous = contiguous && gslot[k] == gslot[0] + int(k);
