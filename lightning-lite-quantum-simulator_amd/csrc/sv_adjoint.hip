@@ -244,7 +244,8 @@ __global__ __launch_bounds__(256) void k_pc_hpsi_tile(const v2d* __restrict__ A,
     const uint32_t tid = threadIdx.x;
     const v2d* psi = A + uint64_t(b) * size;
     v2d* lam = STORE ? Lm + uint64_t(b) * size : nullptr;
-    uint64_t base = blockIdx.x;
+    const uint64_t tix = xcd_tile(blockIdx.x, gridDim.x);  // this workgroup's tile
+    uint64_t base = tix;
     for (int j = 0; j < 12; ++j) base = insert0(base, tl.tq[j]);
     uint32_t glane = 0, greg[16];  // states here have at most 30 qubits: 32-bit indices
     for (int k = 0; k < 8; ++k)
@@ -291,7 +292,7 @@ __global__ __launch_bounds__(256) void k_pc_hpsi_tile(const v2d* __restrict__ A,
         }
     }
     const double sum = bsum(e, lds4);
-    if (tid == 0) partials[(size_t(b) * npass + pass) * gridDim.x + blockIdx.x] = sum;
+    if (tid == 0) partials[(size_t(b) * npass + pass) * gridDim.x + tix] = sum;
 }
 
 // Groups the terms into at most four tile passes (diagonal terms in the first); false when some

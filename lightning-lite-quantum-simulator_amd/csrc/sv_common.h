@@ -47,6 +47,13 @@ inline unsigned grid_for(uint64_t chunks) {
     return unsigned(chunks < (uint64_t(1) << 20) ? (chunks ? chunks : 1) : (uint64_t(1) << 20));
 }
 
+// Tile of workgroup b out of nb: workgroups are dealt round-robin to the 8 XCDs; with a multiple
+// of 8 tiles, XCD k takes the k-th eighth of them in order (measured on a 30-qubit register-tile
+// pass whose tiles are 128 runs of 512 bytes 2 MiB apart: 5.4 against 4.6 TB/s).
+__device__ __forceinline__ uint64_t xcd_tile(uint64_t b, uint64_t nb) {
+    return (nb & 7u) ? b : (b & 7u) * (nb >> 3) + (b >> 3);
+}
+
 __device__ __forceinline__ v2d ld_nt(const v2d* p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ void st_nt(v2d* p, v2d v) { __builtin_nontemporal_store(v, p); }
 

@@ -11,6 +11,9 @@
 // instead of one contiguous run.  Measured on MI355X at 30 qubits, in-place streams in that
 // order run 8-12% faster (6.2-6.6 TB/s against 5.6-6.0 for the contiguous order), which
 // points at DRAM bank parallelism within each workgroup's accesses.
+#include <algorithm>
+#include <vector>
+
 #include "sv_common.h"
 #include "sv_kernels.h"
 
@@ -277,6 +280,78 @@ __global__ __launch_bounds__(256) void k_diag_1q_split(v2d* __restrict__ psi, in
 
 constexpr int kPPT = 4;
 // Lowest qubit whose 2^q-amplitude runs (16 * 2^q bytes) fill whole 128-byte lines.
+// Tile order (states of 18 qubits and more, targets q >= 6): workgroup w takes a tile of 128 runs
+// of 512 bytes (index bits 0..4) 32 KiB apart (bits 11..17) plus the target bit when it lies
+// outside them (then 2^13 amplitudes, 32 per thread); the tile's other index bits are w's, tiles
+// dealt to XCDs in contiguous eighths.  The target is the top register bit: each thread applies the
+// gate to its own pairs (registers j and j + R/2).  Measured on MI355X at 30 qubits with a plain in-place pass over tiles of
+// bits 0..4 and 11..17: 6.7 TB/s, against 5.9 for contiguous 64 KiB tiles.
+struct TileMap {
+    int nb;       // tile bits: 12 or 13
+    int pos[13];  // the tile's index bits, ascending
+    int lane[8];  // index bit of thread bit k
+    int reg[5];   // index bit of register bit k (the last = the gate's target)
+};
+
+constexpr int kTileMinQubits = 18;
+
+TileMap tile_map_1q(int q) {
+    std::vector<int> lanes = {0, 1, 2, 3, 4, 11, 12, 13}, regs = {q};
+    for (int b = 14; b <= 17; ++b)
+        if (b != q) regs.push_back(b);
+    if (q < 11) regs.pop_back();  // 0..4, q, 11..16
+    if (q >= 11 && q <= 13) {      // q is a lane bit of the default map: 17 takes its place
+        for (int& l : lanes)
+            if (l == q) l = 17;
+        regs = {q, 14, 15, 16};
+    }
+    // target last: a thread loads its pairs' lower halves first, then the upper halves
+    std::rotate(regs.begin(), regs.begin() + 1, regs.end());
+    TileMap m{};
+    std::vector<int> all(lanes);
+    all.insert(all.end(), regs.begin(), regs.end());
+    std::sort(all.begin(), all.end());
+    m.nb = int(all.size());
+    for (int k = 0; k < m.nb; ++k) m.pos[k] = all[k];
+    for (int k = 0; k < 8; ++k) m.lane[k] = lanes[k];
+    for (size_t k = 0; k < regs.size(); ++k) m.reg[k] = regs[k];
+    return m;
+}
+
+template <int RB, bool XCD>
+__global__ __launch_bounds__(256, 2) void k_apply_1q_tile(v2d* __restrict__ psi, Gate2 g, TileMap tm) {
+    constexpr int R = 1 << RB;
+    uint64_t base = XCD ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
+#pragma unroll
+    for (int k = 0; k < 8 + RB; ++k) base = insert0(base, tm.pos[k]);
+    uint64_t gt = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if ((threadIdx.x >> k) & 1u) gt |= uint64_t(1) << tm.lane[k];
+    uint64_t rb[RB];
+#pragma unroll
+    for (int k = 0; k < RB; ++k) rb[k] = uint64_t(1) << tm.reg[k];
+    v2d* const p = psi + (base | gt);
+    auto off = [&](int j) {
+        uint64_t o = 0;
+#pragma unroll
+        for (int k = 0; k < RB; ++k)
+            if ((j >> k) & 1) o |= rb[k];
+        return o;
+    };
+    v2d a[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) a[j] = ld_nt(p + off(j));
+#pragma unroll
+    for (int j = 0; j < R / 2; ++j) {
+        const v2d x0 = a[j], x1 = a[j + R / 2];
+        a[j] = row2(g.r, x0, x1);
+        a[j + R / 2] = row2(g.r + 4, x0, x1);
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) st_nt(p + off(j), a[j]);
+}
+
 constexpr int kLineBits = 3;
 
 hipError_t launch_apply_1q(hipStream_t st, v2d* psi, int n, int q, const Gate2& g) {
@@ -292,6 +367,12 @@ hipError_t launch_apply_1q(hipStream_t st, v2d* psi, int n, int q, const Gate2& 
         } else if (diag) {
             const uint64_t nch = 2 * npairs / 512;
             hipLaunchKernelGGL(k_diag_1q_split, dim3(grid_for(nch)), dim3(256), 0, st, psi, q, g, nch, ld);
+        } else if (n >= kTileMinQubits && q >= 6) {
+            const TileMap tm = tile_map_1q(q);
+            if (tm.nb == 13)
+                hipLaunchKernelGGL((k_apply_1q_tile<5, false>), dim3(unsigned(uint64_t(1) << (n - 13))), dim3(256), 0, st, psi, g, tm);
+            else
+                hipLaunchKernelGGL((k_apply_1q_tile<4, true>), dim3(unsigned(uint64_t(1) << (n - 12))), dim3(256), 0, st, psi, g, tm);
         } else if (q < 6) {
             const uint64_t nch = 2 * npairs / 512;
             hipLaunchKernelGGL(k_apply_1q_lane_split, dim3(grid_for(nch)), dim3(256), 0, st, psi, q, g, nch, ld);
@@ -465,8 +546,73 @@ __global__ __launch_bounds__(256) void k_cnot_lane_split(v2d* __restrict__ psi, 
   }
 }
 
+// Tile order for the half-state swap: the tile of k_apply_1q_tile with the target and the control
+// as the two top register bits; a thread loads the control = 1 half of its registers (16
+// amplitudes) and stores each at its partner across the target bit.  Used where control and target
+// are >= 5 and neither is bit 10 or 11 (then thread bit 5 is index bit 11, as in the one-qubit
+// tiles): measured on MI355X at 30 qubits over the CNOT ladder, 2.68-3.10 ms per swap there against
+// 2.69-3.34 for k_cnot_split, while controls 3-4 and 10-11 took 3.1-3.7 ms this way.
+TileMap tile_map_cnot(int c, int t) {
+    constexpr int RB = 5;
+    std::vector<int> lanes, regs;
+    for (int b = 0; b < 5; ++b)
+        if (b != c && b != t) lanes.push_back(b);
+    int b = 11;
+    auto next = [&] {
+        while (b == c || b == t) ++b;
+        return b++;
+    };
+    while (lanes.size() < 8) lanes.push_back(next());
+    while (regs.size() < RB - 2) regs.push_back(next());
+    regs.push_back(t);
+    regs.push_back(c);
+    TileMap m{};
+    std::vector<int> all(lanes);
+    all.insert(all.end(), regs.begin(), regs.end());
+    std::sort(all.begin(), all.end());
+    m.nb = int(all.size());
+    for (int k = 0; k < m.nb; ++k) m.pos[k] = all[k];
+    for (int k = 0; k < 8; ++k) m.lane[k] = lanes[k];
+    for (size_t k = 0; k < regs.size(); ++k) m.reg[k] = regs[k];
+    return m;
+}
+
+__global__ __launch_bounds__(256, 2) void k_cnot_tile(v2d* __restrict__ psi, TileMap tm) {
+    constexpr int RB = 5, H = 1 << (RB - 1), T = 1 << (RB - 2);
+    uint64_t base = blockIdx.x;
+#pragma unroll
+    for (int k = 0; k < 8 + RB; ++k) base = insert0(base, tm.pos[k]);
+    uint64_t gt = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if ((threadIdx.x >> k) & 1u) gt |= uint64_t(1) << tm.lane[k];
+    uint64_t rb[RB - 1];
+#pragma unroll
+    for (int k = 0; k < RB - 1; ++k) rb[k] = uint64_t(1) << tm.reg[k];
+    v2d* const p = psi + (base | gt | (uint64_t(1) << tm.reg[RB - 1]));  // control = 1
+    auto off = [&](int j) {
+        uint64_t o = 0;
+#pragma unroll
+        for (int k = 0; k < RB - 1; ++k)
+            if ((j >> k) & 1) o |= rb[k];
+        return o;
+    };
+    v2d a[H];
+#pragma unroll
+    for (int j = 0; j < H; ++j) a[j] = ld_nt(p + off(j));
+#pragma unroll
+    for (int j = 0; j < H; ++j) st_nt(p + off(j ^ T), a[j]);
+}
+
 hipError_t launch_cnot(hipStream_t st, v2d* psi, int n, int c, int t) {
     const int lo = c < t ? c : t, hi = c < t ? t : c;
+    if (c >= 5 && t >= 5 && c != 10 && c != 11 && t != 10 && t != 11 && n >= kTileMinQubits) {
+        const TileMap tm = tile_map_cnot(c, t);
+        if (tm.pos[tm.nb - 1] < n) {
+            hipLaunchKernelGGL(k_cnot_tile, dim3(unsigned(uint64_t(1) << (n - tm.nb))), dim3(256), 0, st, psi, tm);
+            return hipGetLastError();
+        }
+    }
     if (c >= kLineBits && t >= kLineBits) {
         const int ins[2] = {lo, hi};
         const int d = split_bit(ins, 2);

@@ -259,7 +259,11 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
     if (!gslot.empty()) o.line("__shared__ double gs[%zu][4];", gslot.size());
     o.line("char* const lds = (char*)tile;");
     o.line("const unsigned int tid = threadIdx.x;");
-    o.line("unsigned long long base = blockIdx.x;");
+    // tile of this workgroup: with a multiple of 8 tiles, XCD k (workgroups k, k + 8, ...) takes
+    // the k-th eighth of them in order; gradient partials are stored per tile
+    o.line("const unsigned long long tb = (gridDim.x & 7u) ? blockIdx.x : (blockIdx.x & 7u) * (gridDim.x >> 3) + "
+           "(blockIdx.x >> 3);");
+    o.line("unsigned long long base = tb;");
     o.line("for (int j = 0; j < 12; ++j) base = ((base >> d.pos[j]) << (d.pos[j] + 1)) | (base & ((1ull << d.pos[j]) - 1ull));");
     o.line("unsigned long long lg[12];");
     o.line("#pragma unroll");
@@ -545,7 +549,7 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
             // at run time and the compiler would keep in scratch memory
             o.line("__syncthreads();");
             o.line("for (unsigned int k = tid; k < %zuu; k += 256u)", gslot.size());
-            o.line("partials[((unsigned long long)blockIdx.y * nslot + (%du + k)) * ntiles + blockIdx.x] = "
+            o.line("partials[((unsigned long long)blockIdx.y * nslot + (%du + k)) * ntiles + tb] = "
                    "(gs[k][0] + gs[k][1]) + (gs[k][2] + gs[k][3]);", gslot[0]);
         } else {
             std::string list;
@@ -553,7 +557,7 @@ std::string emit(const RegOp* ops, int nops, const Shape& sh) {
             o.line("{ const int gslot[%zu] = {%s};", gslot.size(), list.c_str());
             o.line("__syncthreads();");
             o.line("for (unsigned int k = tid; k < %zuu; k += 256u)", gslot.size());
-            o.line("partials[((unsigned long long)blockIdx.y * nslot + gslot[k]) * ntiles + blockIdx.x] = "
+            o.line("partials[((unsigned long long)blockIdx.y * nslot + gslot[k]) * ntiles + tb] = "
                    "(gs[k][0] + gs[k][1]) + (gs[k][2] + gs[k][3]); }");
         }
     }

@@ -81,7 +81,7 @@ __global__ __launch_bounds__(NT) void k_pass(v2d* __restrict__ psi, PassDesc d, 
     const uint32_t tsize = 1u << b;
     const uint32_t tid = threadIdx.x;
 
-    uint64_t base = blockIdx.x;
+    uint64_t base = xcd_tile(blockIdx.x, gridDim.x);
     for (int j = 0; j < b; ++j) base = insert0(base, d.pos[j]);
 
     // l = tid + k * NT: the offset splits into a per-thread and a per-iteration part.
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(256) void k_pass_reg(v2d* __restrict__ psi, PassDes
     extern __shared__ v2d tile[];
     char* const lds = reinterpret_cast<char*>(tile);
     const uint32_t tid = threadIdx.x;
-    uint64_t base = blockIdx.x;
+    uint64_t base = xcd_tile(blockIdx.x, gridDim.x);
     for (int j = 0; j < kRegTileBits; ++j) base = insert0(base, d.pos[j]);
 
     // initial layout: register bits = local bits 8..11, thread bits = 0..7 (coalesced runs of 2^L)
