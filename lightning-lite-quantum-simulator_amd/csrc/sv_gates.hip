@@ -280,11 +280,13 @@ __global__ __launch_bounds__(256) void k_diag_1q_split(v2d* __restrict__ psi, in
 
 constexpr int kPPT = 4;
 // Lowest qubit whose 2^q-amplitude runs (16 * 2^q bytes) fill whole 128-byte lines.
-// Tile order (states of 18 qubits and more, targets q >= 6): workgroup w takes a tile of 128 runs
-// of 512 bytes (index bits 0..4) 32 KiB apart (bits 11..17) plus the target bit when it lies
-// outside them (then 2^13 amplitudes, 32 per thread); the tile's other index bits are w's, tiles
-// dealt to XCDs in contiguous eighths.  The target is the top register bit: each thread applies the
-// gate to its own pairs (registers j and j + R/2).  Measured on MI355X at 30 qubits with a plain in-place pass over tiles of
+// Tile order (states of 18 qubits and more): workgroup w takes a tile of 128 runs of 512 bytes
+// (index bits 0..4) 32 KiB apart (bits 11..17), or of bits 0..4, the target and 11..16 for targets
+// 5..9, or of bits 0..4, 11..17 and the target for target 10 and targets >= 18 (2^13 amplitudes, 32
+// per thread); the tile's other index bits are w's, 12-bit tiles dealt to XCDs in contiguous
+// eighths.  A target among the register bits is the top one: each thread applies the gate to its
+// own pairs (registers j and j + R/2); targets 0..4 and 11 are thread bits 0..5, whose pairs meet by
+// a cross-lane exchange.  Measured on MI355X at 30 qubits with a plain in-place pass over tiles of
 // bits 0..4 and 11..17: 6.7 TB/s, against 5.9 for contiguous 64 KiB tiles.
 struct TileMap {
     int nb;       // tile bits: 12 or 13
@@ -295,18 +297,28 @@ struct TileMap {
 
 constexpr int kTileMinQubits = 18;
 
+// Thread bit of the target when it is one (then the gate's pairs meet by a cross-lane exchange
+// within the wavefront), else -1.
+int tile_lane_bit(int q) { return q < 5 ? q : q == 11 ? 5 : -1; }
+
 TileMap tile_map_1q(int q) {
-    std::vector<int> lanes = {0, 1, 2, 3, 4, 11, 12, 13}, regs = {q};
-    for (int b = 14; b <= 17; ++b)
-        if (b != q) regs.push_back(b);
-    if (q < 11) regs.pop_back();  // 0..4, q, 11..16
-    if (q >= 11 && q <= 13) {      // q is a lane bit of the default map: 17 takes its place
+    std::vector<int> lanes = {0, 1, 2, 3, 4, 11, 12, 13}, regs;
+    if (tile_lane_bit(q) >= 0) {
+        regs = {14, 15, 16, 17};  // bits 0..4 and 11..17
+    } else if (q == 12 || q == 13) {  // 17 takes the target's place among the lanes
         for (int& l : lanes)
             if (l == q) l = 17;
-        regs = {q, 14, 15, 16};
+        regs = {14, 15, 16, q};
+    } else if (q >= 14 && q <= 17) {
+        for (int b = 14; b <= 17; ++b)
+            if (b != q) regs.push_back(b);
+        regs.push_back(q);
+    } else if (q < 10) {  // 0..4, q, 11..16
+        regs = {14, 15, 16, q};
+    } else {  // 10 or >= 18: 0..4, 11..17, q (13 bits)
+        regs = {14, 15, 16, 17, q};
     }
-    // target last: a thread loads its pairs' lower halves first, then the upper halves
-    std::rotate(regs.begin(), regs.begin() + 1, regs.end());
+    // a register target is the top register bit: a thread loads its pairs' lower halves first
     TileMap m{};
     std::vector<int> all(lanes);
     all.insert(all.end(), regs.begin(), regs.end());
@@ -318,8 +330,10 @@ TileMap tile_map_1q(int q) {
     return m;
 }
 
-template <int RB, bool XCD>
-__global__ __launch_bounds__(256, 2) void k_apply_1q_tile(v2d* __restrict__ psi, Gate2 g, TileMap tm) {
+// LANE: the target is thread bit qb (< 6, within the wavefront) instead of a register bit; each lane
+// takes its partner's amplitude with a cross-lane exchange.
+template <int RB, bool XCD, bool LANE>
+__global__ __launch_bounds__(256, 2) void k_apply_1q_tile(v2d* __restrict__ psi, Gate2 g, TileMap tm, int qb) {
     constexpr int R = 1 << RB;
     uint64_t base = XCD ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x;
 #pragma unroll
@@ -342,11 +356,24 @@ __global__ __launch_bounds__(256, 2) void k_apply_1q_tile(v2d* __restrict__ psi,
     v2d a[R];
 #pragma unroll
     for (int j = 0; j < R; ++j) a[j] = ld_nt(p + off(j));
+    if (LANE) {
+        const int m = 1 << qb;
+        const bool hi = (threadIdx.x >> qb) & 1;
+        const double* r = hi ? g.r + 4 : g.r;
 #pragma unroll
-    for (int j = 0; j < R / 2; ++j) {
-        const v2d x0 = a[j], x1 = a[j + R / 2];
-        a[j] = row2(g.r, x0, x1);
-        a[j + R / 2] = row2(g.r + 4, x0, x1);
+        for (int j = 0; j < R; ++j) {
+            v2d o;
+            o.x = __shfl_xor(a[j].x, m, 64);
+            o.y = __shfl_xor(a[j].y, m, 64);
+            a[j] = hi ? row2(r, o, a[j]) : row2(r, a[j], o);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < R / 2; ++j) {
+            const v2d x0 = a[j], x1 = a[j + R / 2];
+            a[j] = row2(g.r, x0, x1);
+            a[j + R / 2] = row2(g.r + 4, x0, x1);
+        }
     }
 #pragma unroll
     for (int j = 0; j < R; ++j) st_nt(p + off(j), a[j]);
@@ -367,12 +394,16 @@ hipError_t launch_apply_1q(hipStream_t st, v2d* psi, int n, int q, const Gate2& 
         } else if (diag) {
             const uint64_t nch = 2 * npairs / 512;
             hipLaunchKernelGGL(k_diag_1q_split, dim3(grid_for(nch)), dim3(256), 0, st, psi, q, g, nch, ld);
-        } else if (n >= kTileMinQubits && q >= 6) {
+        } else if (n >= kTileMinQubits) {
             const TileMap tm = tile_map_1q(q);
-            if (tm.nb == 13)
-                hipLaunchKernelGGL((k_apply_1q_tile<5, false>), dim3(unsigned(uint64_t(1) << (n - 13))), dim3(256), 0, st, psi, g, tm);
+            const dim3 grid(unsigned(uint64_t(1) << (n - tm.nb)));
+            const int qb = tile_lane_bit(q);
+            if (qb >= 0)
+                hipLaunchKernelGGL((k_apply_1q_tile<4, true, true>), grid, dim3(256), 0, st, psi, g, tm, qb);
+            else if (tm.nb == 13)
+                hipLaunchKernelGGL((k_apply_1q_tile<5, false, false>), grid, dim3(256), 0, st, psi, g, tm, 0);
             else
-                hipLaunchKernelGGL((k_apply_1q_tile<4, true>), dim3(unsigned(uint64_t(1) << (n - 12))), dim3(256), 0, st, psi, g, tm);
+                hipLaunchKernelGGL((k_apply_1q_tile<4, true, false>), grid, dim3(256), 0, st, psi, g, tm, 0);
         } else if (q < 6) {
             const uint64_t nch = 2 * npairs / 512;
             hipLaunchKernelGGL(k_apply_1q_lane_split, dim3(grid_for(nch)), dim3(256), 0, st, psi, q, g, nch, ld);
