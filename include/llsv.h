@@ -303,9 +303,11 @@ int llsv_traj_run(const llsv_traj_item* items, int n_items, const llsv_tgate* ga
 /* A batch prepared for repeated runs (no counterpart in the reference, whose TrajectorySolver
  * samples one trajectory per call; for callers that re-run a batch with new seeds).  Uploads
  * items, gates and rates once and chooses the launch configuration: threads per trajectory
- * (256, 512 or 1024), persistent workgroups per CU (1, 2 or 4) and whether the noise operator
- * drawn after a gate is applied inside the next one-qubit gate's pass (fusion) or in a pass of
- * its own.  tune = 1: the autotuner times every candidate on a sample of this batch (about
+ * (256, 512 or 1024), persistent workgroups per CU (1, 2 or 4) and the fused mode: 0, every
+ * gate and noise operator applied in a pass of its own; 1, operators of sites that cannot jump
+ * deferred and multiplied into the next pass on their qubit, CNOTs with no possible jump run two
+ * per pass; 2, the same with such CNOTs held in lists that run in one pass through LDS tiles of
+ * the state (at most two workgroups per CU).  tune = 1: the autotuner times every candidate on a sample of this batch (about
  * 2048 items) and keeps the fastest, cached per batch shape (qubit range, log2 of the item
  * count and of the gates per item) for the process, so llsv_traj_run uses it too; tune = 0:
  * the cached choice or the default; tune = -1: the default (llsv_traj_default_config).  One run
@@ -320,12 +322,13 @@ int llsv_traj_plan_create(const llsv_traj_item* items, int n_items, const llsv_t
 int llsv_traj_plan_run(llsv_traj_plan* p, uint64_t seed, double* z, uint32_t* jumps, uint64_t* hashes);
 
 /* The plan's configuration: threads per trajectory, workgroups in the persistent launch,
- * fusion (0 / 1) and the time the autotuner took in ms (0 when the choice came from the cache). */
+ * fused mode (0 / 1 / 2) and the time the autotuner took in ms (0 when the choice came from the cache). */
 int llsv_traj_plan_config(const llsv_traj_plan* p, int* threads, int* workgroups, int* fused, double* tune_ms);
 int llsv_traj_plan_destroy(llsv_traj_plan* p);
 
 /* Sets the process-wide default configuration (initially 256 threads, 4 workgroups per CU,
- * fused): threads in {256, 512, 1024}, per_cu in {1, 2, 4}, threads * per_cu <= 2048. */
+ * fused mode 1): threads in {256, 512, 1024}, per_cu in {1, 2, 4}, threads * per_cu <= 2048,
+ * fused in {0, 1, 2}. */
 int llsv_traj_default_config(int threads, int per_cu, int fused);
 
 /* The tuner's choices (batch shape -> threads, workgroups per CU, fused) as a JSON file, so a

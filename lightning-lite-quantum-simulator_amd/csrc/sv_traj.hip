@@ -174,6 +174,14 @@ __device__ __forceinline__ void acc4(double* v, uint64_t i, int sa, int sb, doub
     v[3] += k == 3 ? p : 0.0;
 }
 
+// Index bits of local index l of a tile over qubit set T: bit j of l at the j-th lowest qubit of T.
+__device__ __forceinline__ uint64_t tile_dep(uint32_t l, uint32_t T) {
+    uint64_t o = 0;
+    for (uint32_t m = T; m && l; m &= m - 1u, l >>= 1)
+        if (l & 1u) o |= uint64_t(m & (~m + 1u));
+    return o;
+}
+
 // Operator ma on qubit a, one pass over the pairs.
 template <int WG, bool STATS>
 __device__ inline void pass_one(v2d* s, uint64_t N, int a, const double* ma, int sa, int sb, double* v) {
@@ -275,6 +283,147 @@ __device__ inline void pass_four(v2d* s, uint64_t N, int c0, const double* mc0, 
     }
 }
 
+// Held CNOTs of a tile pass: at most kMaxHeld, their qubits within one tile of at most kTileQubits.
+constexpr int kMaxHeld = 16, kTileQubits = 12;
+
+// One pass of the state of n qubits through LDS tiles: the held CNOTs h < nh in order (each with
+// its control's and its target's operators before it, hm[h][0] and hm[h][1]: the arithmetic of
+// pass_two), then the operator pend[q] on every qubit q of m1 (two qubits per sweep of the tile),
+// and with `stats` the joint weights of bits (sa, sb) of the result into v (acc4).  The tile set T
+// holds those qubits (at most min(n, kTileQubits), the caller's to ensure) plus the lowest other
+// qubits up to that many bits (local bit j = the j-th lowest qubit of T); each of the
+// 2^(n - |T|) tiles is loaded into `tile` once (full tiles: the next one's loads in flight while
+// the current one is worked on) and goes back once.
+template <int WG>
+__device__ __forceinline__ void tile_ops(v2d* tile, uint32_t ts, uint32_t T, int nh, const int (*hq)[2],
+                                const double (*hm)[2][8], uint32_t m1, const double (*pend)[8]) {
+    const uint32_t tid = threadIdx.x;
+    for (int h = 0; h < nh; ++h) {
+        const int a = __builtin_popcount(T & ((1u << hq[h][0]) - 1u));  // local control
+        const int b = __builtin_popcount(T & ((1u << hq[h][1]) - 1u));  // local target
+        const int lo = a < b ? a : b, hi = a < b ? b : a;
+        const uint32_t ab = 1u << a, bb = 1u << b;
+        double ma[8], mb[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            ma[j] = hm[h][0][j];
+            mb[j] = hm[h][1][j];
+        }
+        for (uint32_t p = tid; p < ts / 4; p += WG) {
+            const uint32_t i = uint32_t(insert0(insert0(p, lo), hi));
+            const v2d x00 = tile[i], x10 = tile[i | ab], x01 = tile[i | bb], x11 = tile[i | ab | bb];
+            const v2d y00 = ap(ma, x00, x10), y10 = ap(ma + 4, x00, x10);
+            const v2d y01 = ap(ma, x01, x11), y11 = ap(ma + 4, x01, x11);
+            const v2d z00 = ap(mb, y00, y01), z01 = ap(mb + 4, y00, y01);
+            const v2d z10 = ap(mb, y10, y11), z11 = ap(mb + 4, y10, y11);
+            tile[i] = z00;
+            tile[i | ab] = z11;  // control set: target flipped
+            tile[i | bb] = z01;
+            tile[i | ab | bb] = z10;
+        }
+        __syncthreads();
+    }
+    while (m1) {
+        const int qa = __builtin_ctz(m1);
+        m1 &= m1 - 1u;
+        const int a = __builtin_popcount(T & ((1u << qa) - 1u));
+        double ma[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) ma[j] = pend[qa][j];
+        if (m1) {  // two operators per sweep (they commute)
+            const int qb = __builtin_ctz(m1);
+            m1 &= m1 - 1u;
+            const int b = __builtin_popcount(T & ((1u << qb) - 1u));
+            double mb[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) mb[j] = pend[qb][j];
+            const int lo = a < b ? a : b, hi = a < b ? b : a;
+            const uint32_t ab = 1u << a, bb = 1u << b;
+            for (uint32_t p = tid; p < ts / 4; p += WG) {
+                const uint32_t i = uint32_t(insert0(insert0(p, lo), hi));
+                const v2d x00 = tile[i], x10 = tile[i | ab], x01 = tile[i | bb], x11 = tile[i | ab | bb];
+                const v2d y00 = ap(ma, x00, x10), y10 = ap(ma + 4, x00, x10);
+                const v2d y01 = ap(ma, x01, x11), y11 = ap(ma + 4, x01, x11);
+                tile[i] = ap(mb, y00, y01);
+                tile[i | bb] = ap(mb + 4, y00, y01);
+                tile[i | ab] = ap(mb, y10, y11);
+                tile[i | ab | bb] = ap(mb + 4, y10, y11);
+            }
+        } else {
+            const uint32_t ab = 1u << a;
+            for (uint32_t p = tid; p < ts / 2; p += WG) {
+                const uint32_t i = uint32_t(insert0(p, a));
+                const v2d x0 = tile[i], x1 = tile[i | ab];
+                tile[i] = ap(ma, x0, x1);
+                tile[i | ab] = ap(ma + 4, x0, x1);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int WG>
+__device__ __forceinline__ void pass_tile(v2d* s, int n, uint32_t hmask, int nh, const int (*hq)[2], const double (*hm)[2][8],
+                                 uint32_t m1, const double (*pend)[8], bool stats, int sa, int sb, double* v,
+                                 v2d* tile) {
+    const int tb = n < kTileQubits ? n : kTileQubits;
+    uint32_t T = hmask | m1;
+    for (int q = 0; q < n && __builtin_popcount(T) < tb; ++q) T |= 1u << q;
+    const uint32_t ts = 1u << tb;
+    const uint64_t ntiles = uint64_t(1) << (n - tb);
+    const uint32_t tid = threadIdx.x;
+    auto tile_base = [&](uint64_t w) {
+        uint64_t base = w;
+        for (int q = 0; q < n; ++q)
+            if ((T >> q) & 1u) base = insert0(base, q);
+        return base;
+    };
+    if (ts < (1u << kTileQubits)) {  // a state of fewer qubits than a full tile: one tile, the whole state
+        for (uint32_t l = tid; l < ts; l += WG) tile[l] = s[tile_dep(l, T)];
+        __syncthreads();
+        tile_ops<WG>(tile, ts, T, nh, hq, hm, m1, pend);
+        for (uint32_t l = tid; l < ts; l += WG) {
+            const uint64_t i = tile_dep(l, T);
+            const v2d x = tile[l];
+            s[i] = x;
+            if (stats) acc4(v, i, sa, sb, nrm2(x));
+        }
+        __syncthreads();
+        return;
+    }
+    // full tiles: element tid + k * WG of a tile is at base | dt | dk[k] (disjoint bits)
+    constexpr int R = (1 << kTileQubits) / WG;
+    const uint64_t dt = tile_dep(tid, T);
+    uint64_t dk[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) dk[k] = tile_dep(uint32_t(k) * WG, T);
+    v2d nxt[R];
+    {
+        const uint64_t b0 = tile_base(0) | dt;
+#pragma unroll
+        for (int k = 0; k < R; ++k) nxt[k] = s[b0 | dk[k]];
+    }
+    for (uint64_t w = 0; w < ntiles; ++w) {
+        const uint64_t base = tile_base(w) | dt;
+#pragma unroll
+        for (int k = 0; k < R; ++k) tile[tid + uint32_t(k) * WG] = nxt[k];
+        __syncthreads();
+        if (w + 1 < ntiles) {
+            const uint64_t bn = tile_base(w + 1) | dt;
+#pragma unroll
+            for (int k = 0; k < R; ++k) nxt[k] = s[bn | dk[k]];
+        }
+        tile_ops<WG>(tile, ts, T, nh, hq, hm, m1, pend);
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const v2d x = tile[tid + uint32_t(k) * WG];
+            s[base | dk[k]] = x;
+            if (stats) acc4(v, base | dk[k], sa, sb, nrm2(x));
+        }
+        __syncthreads();
+    }
+}
+
 // Bits of the possible-jump mask of a gate: site s may jump only if u_s >= 1 - rate (a jump needs
 // u >= 1 - rate * p with p <= 1, both sides rounded the way decide() rounds them).
 __device__ inline unsigned possible_sites(const TG& g, const double* gamma, const double* lam, int first_rate,
@@ -321,19 +470,27 @@ constexpr int TRAJ_CHUNK = 256;  // gates whose possible-jump masks are drawn to
 //     run in one pass (pass_four), else the held one runs alone first.  Operators pushed on a
 //     held CNOT's qubits after it stay pending, so they apply after it; anything that reads or
 //     needs the state runs it first.
+//   * TILE (with DEFER): CNOTs with no possible jump are held in a list instead (s_hq, s_hm: their
+//     qubits and the pending operators of both before each), up to kMaxHeld of them on at most
+//     kTileQubits qubits; the list runs in one pass through LDS tiles of the state (pass_tile)
+//     when the next one would not fit, or before anything reads or needs the state.
 // At a possible jump every thread runs decide() on the same block sums, so the outcome needs no
 // broadcast; the chosen operator, scaled to renormalise, becomes the qubit's pending operator.
 // <Z_q> divides by the final norm.
-template <int WG, bool DEFER>
-__global__ __launch_bounds__(WG) void k_traj(const TItem* __restrict__ items, int n_items,
+template <int WG, bool DEFER, bool TILE = false>
+__global__ __launch_bounds__(WG, TILE && WG == 512 ? 4 : 1) void k_traj(const TItem* __restrict__ items, int n_items,
                                              const int* __restrict__ order, const TG* __restrict__ gates,
                                              const double* __restrict__ gamma, const double* __restrict__ lam,
                                              const uint64_t* __restrict__ seedp, v2d* __restrict__ ws,
                                              uint64_t slot_size, unsigned* __restrict__ counter,
                                              double* __restrict__ zout, unsigned* __restrict__ jout,
                                              uint64_t* __restrict__ hout, int norm_state) {
-    constexpr bool kPair = DEFER && WG <= 512;  // held CNOTs run in pairs (pass_four)
+    constexpr bool kPair = DEFER && WG <= 512 && !TILE;  // held CNOTs run in pairs (pass_four)
+    static_assert(!TILE || DEFER, "tile passes hold deferred CNOTs");
     __shared__ int s_item;
+    __shared__ int s_hq[TILE ? kMaxHeld : 1][2];
+    __shared__ double s_hm[TILE ? kMaxHeld : 1][2][8];
+    extern __shared__ v2d s_tile[];  // TILE: 2^kTileQubits amplitudes (dynamic)
     __shared__ double s_pend[20][8];
     __shared__ unsigned char s_pm[TRAJ_CHUNK];
     __shared__ double lds[8][WG / 64];
@@ -387,18 +544,68 @@ __global__ __launch_bounds__(WG) void k_traj(const TItem* __restrict__ items, in
                 __syncthreads();
             }
         };
-        // s_pend[q] <- a * s_pend[q] (or a when q has nothing pending); thread 0 only.
+        int nh = 0;          // TILE: held CNOTs in s_hq / s_hm
+        uint32_t hmask = 0;  // TILE: their qubits
+        // TILE: the held CNOTs, then the pending operators of the qubits in mask (and with stats the
+        // weights of bits (sa, sb) of the result), in as few tile passes as the tile's qubits allow;
+        // clears mask from pmask.
+        auto flush_tile = [&](uint32_t mask, bool stats, int sa, int sb, double* v) __attribute__((always_inline)) {
+            mask &= pmask;
+            pmask &= ~mask;
+            if (!mask && !nh && !stats) return;
+            const int tb = n < kTileQubits ? n : kTileQubits;
+            do {
+                uint32_t part = 0;
+                for (uint32_t r = mask; r; r &= r - 1u) {
+                    const uint32_t bit = r & (~r + 1u);
+                    if (__builtin_popcount(hmask | part | bit) <= tb) part |= bit;
+                }
+                const bool last = part == mask;
+                __syncthreads();
+                pass_tile<WG>(s, n, hmask, nh, s_hq, s_hm, part, s_pend, stats && last, sa, sb, v, s_tile);
+                nh = 0;
+                hmask = 0;
+                mask &= ~part;
+            } while (mask);
+        };
         // the held CNOT (DEFER), if any: control hc, target ht, their operators before it
         bool hold = false;
         int hc = 0, ht = 0;
         double hmc[8], hmt[8];
         auto run_held = [&] {
+            if (TILE) {
+                flush_tile(0u, false, 0, 0, nullptr);
+                return;
+            }
             if (!hold) return;
             __syncthreads();
             pass_two<WG, false, true>(s, N, hc, hmc, ht, hmt, hc, ht, nullptr);
             __syncthreads();
             hold = false;
         };
+        // TILE: appends CNOT (c, t) with the pending operators of its qubits to the held list
+        // (running the list first when it would not fit).
+        auto hold_cnot = [&](int c, int t) {
+            const uint32_t ct = (1u << c) | (1u << t);
+            if (nh == kMaxHeld || __builtin_popcount(hmask | ct) > (n < kTileQubits ? n : kTileQubits)) run_held();
+            __syncthreads();  // pushes to s_pend visible, the last pass done with s_hm
+            if (tid == 0) {
+                s_hq[nh][0] = c;
+                s_hq[nh][1] = t;
+                for (int e = 0; e < 2; ++e) {
+                    const int q = e ? t : c;
+                    if (pmask & (1u << q)) {
+                        copy8(s_pend[q], s_hm[nh][e]);
+                    } else {
+                        for (int j = 0; j < 8; ++j) s_hm[nh][e][j] = (j == 0 || j == 6) ? 1.0 : 0.0;
+                    }
+                }
+            }
+            ++nh;
+            hmask |= ct;
+            pmask &= ~ct;
+        };
+        // s_pend[q] <- a * s_pend[q] (or a when q has nothing pending); thread 0 only.
         auto push = [&](int q, const double* a) {
             if (tid == 0) {
                 if (pmask & (1u << q))
@@ -430,8 +637,12 @@ __global__ __launch_bounds__(WG) void k_traj(const TItem* __restrict__ items, in
                 } else {
                     push(q, g.m);
                     double v[4] = {0.0, 0.0, 0.0, 0.0}, P4[4];
-                    run_held();
-                    flush(pmask, true, q, q, v);
+                    if (TILE) {
+                        flush_tile(pmask, true, q, q, v);
+                    } else {
+                        run_held();
+                        flush(pmask, true, q, q, v);
+                    }
                     pmask = 0;
                     block_sums<WG, 4>(v, lds, P4);
                     double P[2][2] = {{P4[0], 0.0}, {P4[3], 0.0}};
@@ -448,6 +659,16 @@ __global__ __launch_bounds__(WG) void k_traj(const TItem* __restrict__ items, in
             } else {
                 const int c = g.q0, t = g.q1;
                 const uint32_t ct = (1u << c) | (1u << t);
+                if (TILE && !pm) {
+                    // no possible jump: joins the held list (which runs first if this one does not fit)
+                    hold_cnot(c, t);
+                    double kc[8], kt[8];
+                    no_jump_op(gamma[item.first_rate + c], lam[item.first_rate + c], kc);
+                    no_jump_op(gamma[item.first_rate + t], lam[item.first_rate + t], kt);
+                    push(c, kc);  // thread 0 read s_pend[c], s_pend[t] above; it alone writes them
+                    push(t, kt);
+                    continue;
+                }
                 if (kPair && !pm) {
                     // no possible jump: held for one pass with the next CNOT on two other qubits
                     __syncthreads();
@@ -475,22 +696,29 @@ __global__ __launch_bounds__(WG) void k_traj(const TItem* __restrict__ items, in
                     push(t, kt);
                     continue;
                 }
-                run_held();
-                if (pm) {  // weights after the CNOT need every other pending operator applied first
-                    flush(pmask & ~ct, false, 0, 0, nullptr);
-                    pmask &= ct;
-                }
-                __syncthreads();
-                double mc[8] = {1, 0, 0, 0, 0, 0, 1, 0}, mt[8] = {1, 0, 0, 0, 0, 0, 1, 0};
-                if (pmask & (1u << c)) copy8(s_pend[c], mc);
-                if (pmask & (1u << t)) copy8(s_pend[t], mt);
                 double v[4] = {0.0, 0.0, 0.0, 0.0};
-                if (pm)
-                    pass_two<WG, true, true>(s, N, c, mc, t, mt, c, t, v);
-                else
-                    pass_two<WG, false, true>(s, N, c, mc, t, mt, c, t, v);
-                __syncthreads();
-                pmask &= ~ct;
+                if (TILE) {
+                    // (a possible jump here) the CNOT joins the held list, and one pass runs the
+                    // list, every other pending operator and the weights of (c, t)
+                    hold_cnot(c, t);
+                    flush_tile(pmask, true, c, t, v);
+                } else {
+                    run_held();
+                    if (pm) {  // weights after the CNOT need every other pending operator applied first
+                        flush(pmask & ~ct, false, 0, 0, nullptr);
+                        pmask &= ct;
+                    }
+                    __syncthreads();
+                    double mc[8] = {1, 0, 0, 0, 0, 0, 1, 0}, mt[8] = {1, 0, 0, 0, 0, 0, 1, 0};
+                    if (pmask & (1u << c)) copy8(s_pend[c], mc);
+                    if (pmask & (1u << t)) copy8(s_pend[t], mt);
+                    if (pm)
+                        pass_two<WG, true, true>(s, N, c, mc, t, mt, c, t, v);
+                    else
+                        pass_two<WG, false, true>(s, N, c, mc, t, mt, c, t, v);
+                    __syncthreads();
+                    pmask &= ~ct;
+                }
                 const double gc = gamma[item.first_rate + c], lc = lam[item.first_rate + c];
                 const double gt = gamma[item.first_rate + t], lt = lam[item.first_rate + t];
                 double kc[8], kt[8];
@@ -515,8 +743,12 @@ __global__ __launch_bounds__(WG) void k_traj(const TItem* __restrict__ items, in
                 }
             }
         }
-        run_held();
-        flush(pmask, false, 0, 0, nullptr);
+        if (TILE) {
+            flush_tile(pmask, false, 0, 0, nullptr);
+        } else {
+            run_held();
+            flush(pmask, false, 0, 0, nullptr);
+        }
         pmask = 0;
         // <Z_q> = sum_i p_i (1 - 2 bit_q(i)) / sum_i p_i, seven qubits per read of the state (few
         // accumulators, so the loop stays in registers); the norm comes with the first seven.
@@ -556,7 +788,8 @@ __global__ __launch_bounds__(WG) void k_traj(const TItem* __restrict__ items, in
 struct TrajConfig {
     int wg = 256;       // threads per trajectory (workgroup size)
     int per_cu = 4;     // persistent workgroups per CU
-    int fuse = 1;       // noise operator applied inside the next one-qubit gate's pass
+    int fuse = 1;       // 0: one pass per gate; 1: deferred operators, CNOTs held in pairs;
+                        // 2: deferred operators, CNOTs held in lists run through LDS tiles
 };
 
 int device_cus() {
@@ -577,15 +810,22 @@ hipError_t launch_traj(const TrajConfig& c, int blocks, hipStream_t st, const TI
                        v2d* ws, uint64_t slot, unsigned* counter, double* z, unsigned* j, uint64_t* h,
                        int norm_state = 0) {
     hipLaunchKernelGGL(k_traj_reset, dim3(1), dim3(1), 0, st, counter);
-#define LLSV_TRAJ_LAUNCH(WG, F)                                                                                   \
-    hipLaunchKernelGGL((k_traj<WG, F>), dim3(blocks), dim3(WG), 0, st, items, n_items, order, gates, gamma, lam, \
-                       seed, ws, slot, counter, z, j, h, norm_state)
+#define LLSV_TRAJ_LAUNCH(WG, F, T)                                                                             \
+    hipLaunchKernelGGL((k_traj<WG, F, T>), dim3(blocks), dim3(WG), T ? kTileBytes : 0, st, items, n_items, order, \
+                       gates, gamma, lam, seed, ws, slot, counter, z, j, h, norm_state)
+    constexpr size_t kTileBytes = size_t(16) << kTileQubits;
     if (c.wg == 1024) {
-        if (c.fuse) LLSV_TRAJ_LAUNCH(1024, true); else LLSV_TRAJ_LAUNCH(1024, false);
+        if (c.fuse == 2) LLSV_TRAJ_LAUNCH(1024, true, true);
+        else if (c.fuse) LLSV_TRAJ_LAUNCH(1024, true, false);
+        else LLSV_TRAJ_LAUNCH(1024, false, false);
     } else if (c.wg == 512) {
-        if (c.fuse) LLSV_TRAJ_LAUNCH(512, true); else LLSV_TRAJ_LAUNCH(512, false);
+        if (c.fuse == 2) LLSV_TRAJ_LAUNCH(512, true, true);
+        else if (c.fuse) LLSV_TRAJ_LAUNCH(512, true, false);
+        else LLSV_TRAJ_LAUNCH(512, false, false);
     } else {
-        if (c.fuse) LLSV_TRAJ_LAUNCH(256, true); else LLSV_TRAJ_LAUNCH(256, false);
+        if (c.fuse == 2) LLSV_TRAJ_LAUNCH(256, true, true);
+        else if (c.fuse) LLSV_TRAJ_LAUNCH(256, true, false);
+        else LLSV_TRAJ_LAUNCH(256, false, false);
     }
 #undef LLSV_TRAJ_LAUNCH
     return hipGetLastError();
@@ -602,7 +842,7 @@ struct TrajShape {
 
 bool valid_config(const TrajConfig& c) {
     return (c.wg == 256 || c.wg == 512 || c.wg == 1024) && (c.per_cu == 1 || c.per_cu == 2 || c.per_cu == 4) &&
-           (c.fuse == 0 || c.fuse == 1) && c.wg * c.per_cu <= 2048;
+           (c.fuse == 0 || c.fuse == 1 || c.fuse == 2) && c.wg * c.per_cu <= 2048;
 }
 
 // ---- tuning cache --------------------------------------------------------------------------
@@ -832,8 +1072,9 @@ int traj_upload(TrajDev* d, const llsv_traj_item* items, int n_items, const std:
 }
 
 // The autotuner: times every candidate configuration (threads per trajectory 256 / 512 / 1024,
-// 1 / 2 / 4 persistent workgroups per CU, noise operator fused or in its own pass) on a sample of
-// the batch -- every k-th item, about 2048 in all, largest first -- and returns the fastest.
+// 1 / 2 / 4 persistent workgroups per CU, one pass per gate / deferred operators with CNOTs held
+// in pairs / in LDS tile lists) on a sample of the batch -- every k-th item, about 2048 in all,
+// largest first -- and returns the fastest.
 int traj_tune(TrajDev& d, const llsv_traj_item* items, int n_items, const std::vector<int>& order, int maxn,
               hipStream_t st, TrajConfig* best, double* tune_ms) {
     const int cus = device_cus();
@@ -850,9 +1091,10 @@ int traj_tune(TrajDev& d, const llsv_traj_item* items, int n_items, const std::v
     float best_ms = 1e30f;
     for (int wg : {256, 512, 1024})
         for (int per_cu : {1, 2, 4})
-            for (int fuse : {0, 1}) {
+            for (int fuse : {0, 1, 2}) {
                 if (e != hipSuccess) break;
                 if (wg * per_cu > 2048) continue;  // more threads than one CU holds at once
+                if (fuse == 2 && per_cu > 2) continue;  // 64 KiB LDS tiles: at most two workgroups per CU
                 const TrajConfig c{wg, per_cu, fuse};
                 const int blocks = std::min(traj_blocks(c, int(sample.size()), maxn, cus), d.max_blocks);
                 float ms = 0.0f;
@@ -1040,7 +1282,7 @@ extern "C" int llsv_traj_default_config(int threads, int per_cu, int fused) {
     using namespace llsv;
     if (!valid_config(TrajConfig{threads, per_cu, fused}))
         return fail(LLSV_ERR_ARG, "llsv_traj_default_config: threads in {256, 512, 1024}, per_cu in {1, 2, 4}, "
-                                  "threads * per_cu <= 2048, fused in {0, 1}");
+                                  "threads * per_cu <= 2048, fused in {0, 1, 2}");
     std::lock_guard<std::mutex> lock(g_tune_mu);
     (void)tune_cache();  // the environment's settings first, so this call overrides them
     g_default = TrajConfig{threads, per_cu, fused};

@@ -67,9 +67,11 @@ def _pack_batch(circuits):
     return items, gates, np.asarray(gam, np.float64), np.asarray(lam, np.float64)
 
 
-def set_default_config(threads: int, per_cu: int, fused: bool):
-    """Process-wide default launch configuration (llsv_traj_default_config)."""
-    _native.check(_native.lib().llsv_traj_default_config(threads, per_cu, int(bool(fused))))
+def set_default_config(threads: int, per_cu: int, fused):
+    """Process-wide default launch configuration (llsv_traj_default_config); fused: 0 / False (one
+    pass per gate), 1 / True (deferred operators, CNOTs held in pairs), 2 (deferred operators,
+    CNOTs held in lists that run through LDS tiles)."""
+    _native.check(_native.lib().llsv_traj_default_config(threads, per_cu, int(fused)))
 
 
 def load_tuning(path: str):
@@ -121,11 +123,11 @@ class TrajectoryPlan:
                                               int(self._gam.size), int(tune), ctypes.byref(self._h)))
 
     def config(self):
-        """(threads per trajectory, workgroups, fused, tuning ms)."""
+        """(threads per trajectory, workgroups, fused mode 0 / 1 / 2, tuning ms)."""
         t, w, f, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         _native.check(self._lib.llsv_traj_plan_config(self._h, ctypes.byref(t), ctypes.byref(w), ctypes.byref(f),
                                                       ctypes.byref(ms)))
-        return t.value, w.value, bool(f.value), ms.value
+        return t.value, w.value, f.value, ms.value
 
     def run(self, seed: int):
         """(z (items, 32), jumps, hashes) for this seed."""

@@ -25,12 +25,12 @@ def write(path, entries, fmt="llsv-traj-tune-1"):
 def test_load_save_roundtrip(tmp_path):
     from llsv import trajectory as LT
 
-    a = [entry(14, 14, 12, 8, 512, 2, 1), entry(16, 10, 11, 7, 1024, 1, 0)]
+    a = [entry(14, 14, 12, 8, 512, 2, 1), entry(16, 10, 11, 7, 1024, 1, 0), entry(13, 13, 12, 8, 256, 2, 2)]
     src = tmp_path / "a.json"
     write(src, a)
     before = LT.tuning_entries()
     LT.load_tuning(str(src))
-    assert LT.tuning_entries() >= max(before, 2)
+    assert LT.tuning_entries() >= max(before, 3)
     out = tmp_path / "b.json"
     LT.save_tuning(str(out))
     saved = json.load(open(out))
@@ -50,6 +50,7 @@ def test_load_save_roundtrip(tmp_path):
     {"format": "llsv-traj-tune-1", "entries": [entry(14, 14, 12, 8, 300, 2, 1)]},   # threads not offered
     {"format": "llsv-traj-tune-1", "entries": [entry(14, 14, 12, 8, 1024, 4, 1)]},  # 4096 threads per CU
     {"format": "llsv-traj-tune-1", "entries": [entry(22, 14, 12, 8, 256, 1, 1)]},   # above 20 qubits
+    {"format": "llsv-traj-tune-1", "entries": [entry(14, 14, 12, 8, 512, 2, 3)]},   # no such fused mode
     {"format": "llsv-traj-tune-1", "entries": [{"maxn": 14}]},                      # fields missing
 ])
 def test_malformed_files_are_refused(tmp_path, bad):
