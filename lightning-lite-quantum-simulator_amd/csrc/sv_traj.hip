@@ -294,6 +294,10 @@ constexpr int kMaxHeld = 16, kTileQubits = 12;
 // qubits up to that many bits (local bit j = the j-th lowest qubit of T); each of the
 // 2^(n - |T|) tiles is loaded into `tile` once (full tiles: the next one's loads in flight while
 // the current one is worked on) and goes back once.
+// LDS slot of tile element l: l ^ (bits 4..7 of l), an XOR-linear bijection on any tile of 16 or
+// more elements that spreads the strided accesses of sweeps over low bits across the banks.
+__device__ __forceinline__ uint32_t tsw(uint32_t l) { return l ^ ((l >> 4) & 15u); }
+
 template <int WG>
 __device__ __forceinline__ void tile_ops(v2d* tile, uint32_t ts, uint32_t T, int nh, const int (*hq)[2],
                                 const double (*hm)[2][8], uint32_t m1, const double (*pend)[8]) {
@@ -309,17 +313,18 @@ __device__ __forceinline__ void tile_ops(v2d* tile, uint32_t ts, uint32_t T, int
             ma[j] = hm[h][0][j];
             mb[j] = hm[h][1][j];
         }
+        const uint32_t sa = tsw(ab), sb = tsw(bb), sab = sa ^ sb;
         for (uint32_t p = tid; p < ts / 4; p += WG) {
-            const uint32_t i = uint32_t(insert0(insert0(p, lo), hi));
-            const v2d x00 = tile[i], x10 = tile[i | ab], x01 = tile[i | bb], x11 = tile[i | ab | bb];
+            const uint32_t i = tsw(uint32_t(insert0(insert0(p, lo), hi)));
+            const v2d x00 = tile[i], x10 = tile[i ^ sa], x01 = tile[i ^ sb], x11 = tile[i ^ sab];
             const v2d y00 = ap(ma, x00, x10), y10 = ap(ma + 4, x00, x10);
             const v2d y01 = ap(ma, x01, x11), y11 = ap(ma + 4, x01, x11);
             const v2d z00 = ap(mb, y00, y01), z01 = ap(mb + 4, y00, y01);
             const v2d z10 = ap(mb, y10, y11), z11 = ap(mb + 4, y10, y11);
             tile[i] = z00;
-            tile[i | ab] = z11;  // control set: target flipped
-            tile[i | bb] = z01;
-            tile[i | ab | bb] = z10;
+            tile[i ^ sa] = z11;  // control set: target flipped
+            tile[i ^ sb] = z01;
+            tile[i ^ sab] = z10;
         }
         __syncthreads();
     }
@@ -339,23 +344,25 @@ __device__ __forceinline__ void tile_ops(v2d* tile, uint32_t ts, uint32_t T, int
             for (int j = 0; j < 8; ++j) mb[j] = pend[qb][j];
             const int lo = a < b ? a : b, hi = a < b ? b : a;
             const uint32_t ab = 1u << a, bb = 1u << b;
+            const uint32_t sa = tsw(ab), sb = tsw(bb), sab = sa ^ sb;
             for (uint32_t p = tid; p < ts / 4; p += WG) {
-                const uint32_t i = uint32_t(insert0(insert0(p, lo), hi));
-                const v2d x00 = tile[i], x10 = tile[i | ab], x01 = tile[i | bb], x11 = tile[i | ab | bb];
+                const uint32_t i = tsw(uint32_t(insert0(insert0(p, lo), hi)));
+                const v2d x00 = tile[i], x10 = tile[i ^ sa], x01 = tile[i ^ sb], x11 = tile[i ^ sab];
                 const v2d y00 = ap(ma, x00, x10), y10 = ap(ma + 4, x00, x10);
                 const v2d y01 = ap(ma, x01, x11), y11 = ap(ma + 4, x01, x11);
                 tile[i] = ap(mb, y00, y01);
-                tile[i | bb] = ap(mb + 4, y00, y01);
-                tile[i | ab] = ap(mb, y10, y11);
-                tile[i | ab | bb] = ap(mb + 4, y10, y11);
+                tile[i ^ sb] = ap(mb + 4, y00, y01);
+                tile[i ^ sa] = ap(mb, y10, y11);
+                tile[i ^ sab] = ap(mb + 4, y10, y11);
             }
         } else {
             const uint32_t ab = 1u << a;
+            const uint32_t sa = tsw(ab);
             for (uint32_t p = tid; p < ts / 2; p += WG) {
-                const uint32_t i = uint32_t(insert0(p, a));
-                const v2d x0 = tile[i], x1 = tile[i | ab];
+                const uint32_t i = tsw(uint32_t(insert0(p, a)));
+                const v2d x0 = tile[i], x1 = tile[i ^ sa];
                 tile[i] = ap(ma, x0, x1);
-                tile[i | ab] = ap(ma + 4, x0, x1);
+                tile[i ^ sa] = ap(ma + 4, x0, x1);
             }
         }
         __syncthreads();
@@ -379,12 +386,12 @@ __device__ __forceinline__ void pass_tile(v2d* s, int n, uint32_t hmask, int nh,
         return base;
     };
     if (ts < (1u << kTileQubits)) {  // a state of fewer qubits than a full tile: one tile, the whole state
-        for (uint32_t l = tid; l < ts; l += WG) tile[l] = s[tile_dep(l, T)];
+        for (uint32_t l = tid; l < ts; l += WG) tile[tsw(l)] = s[tile_dep(l, T)];
         __syncthreads();
         tile_ops<WG>(tile, ts, T, nh, hq, hm, m1, pend);
         for (uint32_t l = tid; l < ts; l += WG) {
             const uint64_t i = tile_dep(l, T);
-            const v2d x = tile[l];
+            const v2d x = tile[tsw(l)];
             s[i] = x;
             if (stats) acc4(v, i, sa, sb, nrm2(x));
         }
@@ -406,7 +413,7 @@ __device__ __forceinline__ void pass_tile(v2d* s, int n, uint32_t hmask, int nh,
     for (uint64_t w = 0; w < ntiles; ++w) {
         const uint64_t base = tile_base(w) | dt;
 #pragma unroll
-        for (int k = 0; k < R; ++k) tile[tid + uint32_t(k) * WG] = nxt[k];
+        for (int k = 0; k < R; ++k) tile[tsw(tid + uint32_t(k) * WG)] = nxt[k];
         __syncthreads();
         if (w + 1 < ntiles) {
             const uint64_t bn = tile_base(w + 1) | dt;
@@ -416,7 +423,7 @@ __device__ __forceinline__ void pass_tile(v2d* s, int n, uint32_t hmask, int nh,
         tile_ops<WG>(tile, ts, T, nh, hq, hm, m1, pend);
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            const v2d x = tile[tid + uint32_t(k) * WG];
+            const v2d x = tile[tsw(tid + uint32_t(k) * WG)];
             s[base | dk[k]] = x;
             if (stats) acc4(v, base | dk[k], sa, sb, nrm2(x));
         }
