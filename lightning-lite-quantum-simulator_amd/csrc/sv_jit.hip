@@ -685,11 +685,8 @@ void compile_batch(const std::vector<JitEntry*>& batch) {
     hiprtcResult r = hiprtcCreateProgram(&prog, src.c_str(), "llsv_pass.hip", 0, nullptr, nullptr);
     if (r == HIPRTC_SUCCESS) {
         // CodeGenPrepare and the SLP vectorizer take half the compile time of these long
-        // straight-line kernels and change nothing in the code they produce.  The iterative ILP
-        // scheduler keeps more independent FP64 work in flight at the same register budget
-        // (measured: adjoint batch 22.1 -> 21.3 ms, the 30-qubit sweep's program step 23.9 -> 23.6 ms).
+        // straight-line kernels and change nothing in the code they produce.
         const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-fno-slp-vectorize", "-mllvm", "-disable-cgp",
-                              "-mllvm", "-amdgpu-sched-strategy=iterative-ilp",
                               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
         r = hiprtcCompileProgram(prog, int(sizeof opts / sizeof opts[0]), opts);
         size_t n = 0;
